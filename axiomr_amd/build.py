@@ -1,0 +1,68 @@
+"""Build the native library of the render path for gfx950.
+
+Compiles the HIP kernels and the C-ABI host code with hipcc into
+``axiomr_amd/libaxiomr_amd.so`` (in-tree, so it travels with the source tree).
+Every translation unit is compiled with FP contraction off and packed-FP32
+instructions disabled; denormals are preserved.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+ROOT = PKG.parent
+BUILD = PKG / "_build"
+LIB = PKG / "libaxiomr_amd.so"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off", "-fno-fast-math",
+    f"-I{ROOT / 'include'}", f"-I{CSRC}",
+    "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
+]
+DEVICE = [
+    f"--offload-arch={ARCH}",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+SOURCES = ["raster.hip", "capi.hip", "scene.cpp"]
+
+
+def _compile(src: str) -> Path:
+    out = BUILD / (Path(src).stem + ".o")
+    path = CSRC / src
+    if src.endswith(".hip"):
+        cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", str(path), "-o", str(out)]
+    else:
+        cmd = [HIPCC, "-x", "c++", *COMMON, "-c", str(path), "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def build(verbose: bool = False) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

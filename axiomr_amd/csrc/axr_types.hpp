@@ -1,0 +1,165 @@
+// axr_types.hpp - plain data shared by the host side of the library and the
+// gfx950 kernels: mesh vertices, materials, per-view draw descriptors, the
+// triangle record, and the frame-arena layout.
+#pragma once
+#include <cstdint>
+#include "axr_math.hpp"
+
+namespace axr {
+
+// Mesh vertex, 14 floats = 56 bytes, same field order as the reference's
+// Vertex (reference: include/mesh.hpp:9-18).
+struct alignas(8) Vertex56 {
+    float px, py, pz;
+    float u, v;
+    float nx, ny, nz;
+    float tx, ty, tz;
+    float bx, by, bz;
+};
+static_assert(sizeof(Vertex56) == 56, "vertex layout");
+
+// One RGBA8 texture resident in HBM: texel (x, y) is the uint32 at
+// texels[y * w + x], red in the low byte.
+struct DevTexture {
+    const uint32_t* texels;
+    int32_t w, h;
+    int32_t filter;  // 0 nearest (the reference's sampling), 1 bilinear
+    int32_t pad;
+};
+
+// Reference Material (include/mesh.hpp:20-34) as far as the shaders read it.
+struct DevMaterial {
+    DevTexture diffuse, metallic, bump, roughness, ao;
+    float specular_exponent;
+    float pad[3];
+};
+
+// One material group in submission order: faces [first, first + count) of
+// the index buffer are submission ranks [rank_base, rank_base + count).
+struct DevGroup {
+    uint32_t first;
+    uint32_t count;
+    uint32_t rank_base;
+    uint32_t material;
+};
+
+enum ShaderKind : int32_t {
+    SHADER_FLAT = 0,
+    SHADER_PHONG = 1,
+    SHADER_PBR = 2,
+    SHADER_GOURAUD = 3,
+    SHADER_PHONG_POINT = 4,
+    SHADER_FLAT_COLOR = 5,
+    SHADER_COUNT = 6,
+};
+
+constexpr int MAX_POINT_LIGHTS = 4;
+
+// Per-draw uniforms of the shaders (reference: include/IShader.hpp:39-45 plus
+// the public fields of the shaders in include/shaders/shaders.hpp).
+struct Uniforms {
+    mat4 model;
+    mat3 normal_mat;      // mat3(transpose(inverse(model))), for the flat shader
+    vec3 camera_pos;
+    vec3 light_dir;
+    vec3 light_color;
+    int32_t kind;
+    int32_t num_lights;
+    vec3 light_pos[MAX_POINT_LIGHTS];
+    vec3 light_colors[MAX_POINT_LIGHTS];
+};
+
+// Everything the kernels need to know about one view (one drawMesh into one
+// framebuffer).  A batch is an array of these; the plain path uses one.
+struct ViewDesc {
+    // mesh
+    const float* pos;              // 3 floats per vertex (packed positions)
+    const Vertex56* verts;         // full vertices, read only for shading winners
+    const uint32_t* idx;           // 3 per face
+    const uint32_t* face_colors;   // RGBA8 per face or null
+    const DevGroup* groups;
+    const DevMaterial* materials;
+    uint32_t ngroups;
+    uint32_t nfaces;               // faces in submission order (sum of group counts)
+    // transform
+    mat4 mvp;
+    Uniforms u;
+    // framebuffer
+    uint32_t* color;               // BGRA8 packed, row 0 = bottom
+    float* depth;
+    uint32_t* ids;                 // optional winner keys
+    int32_t W, H;
+    int32_t tiles_x, tiles_y;
+    // offsets of this view inside the batch-wide arrays
+    uint32_t face_base;            // first slot of this view in the record array
+    uint32_t tile_base;            // first bin of this view
+};
+
+// Triangle record, 48 bytes (three 16-byte loads).
+//   a = (x0, y0, x1, y1)      screen position, y up
+//   b = (x2, y2, z0, z1)      ndc z
+//   c = (bits(z2), key, ref, info)
+// key  = submission rank * 8 + clip sub-triangle: the depth tie-break order.
+// For a record in the face-slot array, ref is the mesh face index; for a
+// record in the clip array, ref is the view index.  A face slot whose face
+// was clipped holds a redirect instead: info has REC_CLIPPED set and ref is
+// the first clip record of that face (sub-triangle j sits at ref + j).
+struct alignas(16) TriRec {
+    float4 a;
+    float4 b;
+    uint4 c;
+};
+constexpr uint32_t REC_CLIPPED = 1u << 31;
+constexpr uint32_t REC_MATERIAL_MASK = 0xFFFFu;
+constexpr uint32_t REF_CLIP = 1u << 31;     // bin entry / binnable ref names a clip record
+
+// Counter block at the head of the frame arena.
+enum Counter : uint32_t {
+    CNT_CLIP_FACES = 0,   // faces queued for the full clipper
+    CNT_CLIP_RECS = 1,    // clip records reserved
+    CNT_BINNABLE = 2,     // records that reach at least one bin
+    CNT_PAIRS = 3,        // (tile, record) pairs
+    CNT_ERROR = 4,        // sticky error bits, see ERR_*
+    CNT_NEED_CLIP_FACES = 5,
+    CNT_NEED_CLIP_RECS = 6,
+    CNT_NEED_PAIRS = 7,
+    CNT_COUNT = 16,
+};
+constexpr uint32_t ERR_CLIP_FACES = 1u;
+constexpr uint32_t ERR_CLIP_RECS = 2u;
+constexpr uint32_t ERR_PAIRS = 4u;
+
+// Device frame arena: the per-frame working memory of the binning pipeline.
+struct Arena {
+    uint32_t* counters;        // CNT_COUNT words
+    TriRec* rec;               // one slot per face of every view
+    uint32_t* clip_faces;      // global face slots queued for clipping
+    TriRec* clip_rec;          // clipped sub-triangles
+    Vertex56* clip_verts;      // 3 per clip record
+    uint32_t* binnable;        // refs of records that reach a bin
+    uint32_t* tile_count;      // per bin
+    uint32_t* tile_off;        // exclusive prefix of tile_count (+1 entry)
+    uint32_t* tile_cursor;     // scatter cursors
+    uint32_t* bins;            // refs, grouped by bin
+    uint32_t* scan_sums;       // per-scan-block partial sums
+    uint32_t cap_clip_faces;
+    uint32_t cap_clip_recs;
+    uint32_t cap_binnable;
+    uint32_t cap_pairs;
+};
+
+// Kernel arguments common to every stage.  `table` points to `nviews`
+// descriptors in device memory; when it is null the single descriptor `one`
+// (passed by value) is the only view.
+struct KArgs {
+    const ViewDesc* table;
+    const uint32_t* face_bases;   // nviews + 1 prefix (device), batch only
+    const uint32_t* tile_bases;   // nviews + 1 prefix (device), batch only
+    uint32_t nviews;
+    uint32_t total_faces;
+    uint32_t total_tiles;
+    int32_t tile_size;
+    ViewDesc one;
+};
+
+}  // namespace axr

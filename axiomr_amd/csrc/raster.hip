@@ -1,0 +1,762 @@
+// raster.hip - the gfx950 kernels of the tiled render path.
+//
+// Stage map (reference: src/tiled_pipeline.cpp:143-322 drawMesh):
+//   k_prep        zero the per-frame counters and bin counters
+//   k_setup       one lane per face: transform, frustum classification, cull,
+//                 triangle setup, bin counting           (tpl.cpp:185-240, 53-119)
+//   k_clip        faces that cross a frustum plane: the full 6-plane clipper
+//                 (pipeline.cpp:176-370), then cull/setup/count as above
+//   k_scan_*      exclusive prefix sum of the bin counts (the frame arena layout)
+//   k_scatter     write every (bin, record) pair          (tpl.cpp:368-393)
+//   k_raster      one workgroup per bin: coverage and early-Z into LDS keys,
+//                 then per-pixel resolve: depth test against the framebuffer,
+//                 shading of the winner, BGRA/depth store  (tpl.cpp:395-594,
+//                 1125-1181)
+//
+// Numerics: every coverage and depth value is computed with the reference's
+// FP32 formulas in its evaluation order (no contraction), anchored at the
+// reference's 16-pixel tiles and 8-pixel steps whatever the bin size here.
+// The per-pixel winner is the minimum of the 64-bit key
+// (ordered depth, submission key), which equals the reference's first-wins
+// strict-less depth test over triangles in submission order.
+#include "axr_types.hpp"
+#include "shade.hpp"
+
+#pragma clang fp contract(off)
+
+namespace axr {
+
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_ITEMS;
+
+// ---------------------------------------------------------------------------
+// helpers
+
+template <bool BATCH>
+__device__ __forceinline__ const ViewDesc& view_at(const KArgs& ka, uint32_t v) {
+    if constexpr (BATCH) return ka.table[v];
+    else return ka.one;
+}
+
+// Largest i in [0, n) with a[i] <= x (a ascending, a[0] <= x).
+__device__ __forceinline__ uint32_t find_le(const uint32_t* a, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ DevGroup group_of(const ViewDesc& V, uint32_t rank) {
+    uint32_t lo = 0, hi = V.ngroups;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (V.groups[mid].rank_base <= rank) lo = mid; else hi = mid;
+    }
+    return V.groups[lo];
+}
+
+__device__ __forceinline__ vec3 load_pos(const float* p, uint32_t i) {
+    return make3(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]);
+}
+
+// Wave-aggregated append: returns this lane's slot for lanes with pred set.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    uint32_t base = 0;
+    int leader = __ffsll((long long)mask) - 1;
+    if (pred && (int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return base + (uint32_t)__popcll(mask & __lanemask_lt());
+}
+
+__device__ __forceinline__ void set_error(const Arena& ar, uint32_t bit, uint32_t need_slot, uint32_t need) {
+    atomicOr(&ar.counters[CNT_ERROR], bit);
+    atomicMax(&ar.counters[need_slot], need);
+}
+
+// reference tiled_pipeline.cpp:15-23
+__device__ __forceinline__ float clampW(float w) {
+    const float tiny = 1e-6f;
+    if (fabsf(w) < tiny) w = (w < 0.0f) ? -tiny : tiny;
+    return w;
+}
+
+// reference pipeline.cpp:274-285
+__device__ __forceinline__ float plane_dist(const vec4& v, int plane) {
+    switch (plane) {
+    case 0: return v.x + v.w;
+    case 1: return v.w - v.x;
+    case 2: return v.y + v.w;
+    case 3: return v.w - v.y;
+    case 4: return v.z + v.w;
+    default: return v.w - v.z;
+    }
+}
+
+enum { CLS_INSIDE = 0, CLS_DROP = 1, CLS_CLIP = 2 };
+// The clipper's result for the cases it decides without interpolating: a
+// triangle inside every plane passes through unchanged, and one entirely
+// outside the first plane it is not entirely inside is dropped.  Every other
+// triangle goes through the full clipper.
+__device__ __forceinline__ int classify(const vec4& a, const vec4& b, const vec4& c) {
+    for (int p = 0; p < 6; ++p) {
+        float d0 = plane_dist(a, p), d1 = plane_dist(b, p), d2 = plane_dist(c, p);
+        if (d0 >= 0.f && d1 >= 0.f && d2 >= 0.f) continue;
+        if (d0 < 0.f && d1 < 0.f && d2 < 0.f) return CLS_DROP;
+        return CLS_CLIP;
+    }
+    return CLS_INSIDE;
+}
+
+// Triangle setup: screen position (y up) and ndc z of one corner
+// (reference tiled_pipeline.cpp:59-65).
+__device__ __forceinline__ void to_screen(const vec4& c, int W, int H, float& sx, float& sy, float& z) {
+    float invW = 1.0f / clampW(c.w);
+    z = c.z * invW;
+    sx = (((c.x * invW) + 1.0f) * 0.5f) * (float)W;
+    sy = (((c.y * invW) + 1.0f) * 0.5f) * (float)H;
+}
+
+// reference tiled_pipeline.cpp:89-119
+__device__ __forceinline__ bool is_backface(const TriRec& r) {
+    float dx1 = r.a.z - r.a.x, dy1 = r.a.w - r.a.y;
+    float dx2 = r.b.x - r.a.x, dy2 = r.b.y - r.a.y;
+    return (dx1 * dy2 - dx2 * dy1) < 0;
+}
+
+__device__ __forceinline__ TriRec make_rec(const vec4& c0, const vec4& c1, const vec4& c2, int W, int H) {
+    TriRec r;
+    float z0, z1, z2;
+    to_screen(c0, W, H, r.a.x, r.a.y, z0);
+    to_screen(c1, W, H, r.a.z, r.a.w, z1);
+    to_screen(c2, W, H, r.b.x, r.b.y, z2);
+    r.b.z = z0; r.b.w = z1;
+    r.c.x = __float_as_uint(z2);
+    return r;
+}
+
+// Edge functions of one triangle as the reference raster prepares them
+// (tiled_pipeline.cpp:436-487), plus its pixel bounds.
+struct RasterTri {
+    float a0, b0, c0, a1, b1, c1, a2, b2, c2;
+    float inv_area;
+    float z0, z1, z2;
+    int fx0, fy0, cx1, cy1;   // (int)floor(min), (int)ceil(max)
+};
+
+__device__ __forceinline__ bool raster_setup(const TriRec& r, RasterTri& t) {
+    const float x0 = r.a.x, y0 = r.a.y, x1 = r.a.z, y1 = r.a.w, x2 = r.b.x, y2 = r.b.y;
+    t.fx0 = cvt_i32(floorf(smin3(x0, x1, x2)));
+    t.fy0 = cvt_i32(floorf(smin3(y0, y1, y2)));
+    t.cx1 = cvt_i32(ceilf(smax3(x0, x1, x2)));
+    t.cy1 = cvt_i32(ceilf(smax3(y0, y1, y2)));
+    float c0 = x1 * y2 - x2 * y1;
+    float c1 = x2 * y0 - x0 * y2;
+    float c2 = x0 * y1 - x1 * y0;
+    float area = (c0 + c1) + c2;
+    if (area >= 0 && (double)area < 1.0e-12) return false;
+    float a0 = y1 - y2, b0 = x2 - x1;
+    float a1 = y2 - y0, b1 = x0 - x2;
+    float a2 = y0 - y1, b2 = x1 - x0;
+    if (area < 0) {
+        a0 = -a0; b0 = -b0; c0 = -c0;
+        a1 = -a1; b1 = -b1; c1 = -c1;
+        a2 = -a2; b2 = -b2; c2 = -c2;
+        area = -area;
+    }
+    t.a0 = a0; t.b0 = b0; t.c0 = c0;
+    t.a1 = a1; t.b1 = b1; t.c1 = c1;
+    t.a2 = a2; t.b2 = b2; t.c2 = c2;
+    t.inv_area = 1.0f / area;
+    t.z0 = r.b.z; t.z1 = r.b.w; t.z2 = __uint_as_float(r.c.x);
+    return true;
+}
+
+// Pixel rectangle [x0,x1) x [y0,y1) that the reference raster visits for
+// this triangle over the whole screen.
+struct Rect { int x0, y0, x1, y1; };
+__device__ __forceinline__ Rect screen_rect(const RasterTri& t, int W, int H) {
+    return Rect{imax(0, t.fx0), imax(0, t.fy0), imin(W, t.cx1), imin(H, t.cy1)};
+}
+
+// Coverage and depth of pixel (px, py), reference tiled_pipeline.cpp:502-562:
+// the row value is anchored at sx = max(16*(px/16), floor(minX)); pixels 8..15
+// past the anchor take one +8*a step; then + a*lane.
+__device__ __forceinline__ bool cover(const RasterTri& t, int px, int py, float& al, float& be, float& ga, float& z) {
+    const int sx = imax(px & ~15, t.fx0);
+    const float sxc = (float)sx + 0.5f;
+    const float pyc = (float)py + 0.5f;
+    float r0 = (t.a0 * sxc + t.b0 * pyc) + t.c0;
+    float r1 = (t.a1 * sxc + t.b1 * pyc) + t.c1;
+    float r2 = (t.a2 * sxc + t.b2 * pyc) + t.c2;
+    const int d = px - sx;
+    if (d >= 8) {
+        r0 = r0 + t.a0 * 8.0f;
+        r1 = r1 + t.a1 * 8.0f;
+        r2 = r2 + t.a2 * 8.0f;
+    }
+    const float lane = (float)(d & 7);
+    const float e0 = r0 + t.a0 * lane;
+    const float e1 = r1 + t.a1 * lane;
+    const float e2 = r2 + t.a2 * lane;
+    if (!(e0 >= 0.f && e1 >= 0.f && e2 >= 0.f)) return false;
+    al = e0 * t.inv_area;
+    be = e1 * t.inv_area;
+    ga = e2 * t.inv_area;
+    z = (t.z0 * al + t.z1 * be) + t.z2 * ga;
+    return true;
+}
+
+// 64-bit depth key: order-preserving image of z (with -0 folded onto +0, the
+// reference's strict-less test treats them as equal) above the submission key.
+__device__ __forceinline__ unsigned long long depth_key(float z, uint32_t key) {
+    uint32_t u = __float_as_uint(z);
+    if ((u << 1) == 0) u = 0;
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ((unsigned long long)u << 32) | key;
+}
+constexpr unsigned long long EMPTY_KEY = ~0ull;
+
+// ---------------------------------------------------------------------------
+// k_prep: zero per-frame counters (the sticky error word and the "need"
+// high-water marks are left alone) and the bin counters/cursors.
+
+__global__ void k_prep(Arena ar, uint32_t ntiles) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < CNT_ERROR) ar.counters[tid] = 0;
+    for (uint32_t i = tid; i < ntiles; i += gridDim.x * blockDim.x) {
+        ar.tile_count[i] = 0;
+        ar.tile_cursor[i] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// binning helpers
+
+template <int T>
+__device__ __forceinline__ void count_bins(const Arena& ar, const ViewDesc& V, const Rect& rc) {
+    const int tx0 = rc.x0 / T, tx1 = (rc.x1 - 1) / T;
+    const int ty0 = rc.y0 / T, ty1 = (rc.y1 - 1) / T;
+    for (int ty = ty0; ty <= ty1; ++ty)
+        for (int tx = tx0; tx <= tx1; ++tx)
+            atomicAdd(&ar.tile_count[V.tile_base + (uint32_t)(ty * V.tiles_x + tx)], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// k_setup: one lane per face (submission order across views).
+
+template <int T, bool BATCH>
+__global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (ar.counters[CNT_ERROR]) return;
+    bool active = g < ka.total_faces;
+    bool want_clip = false, binnable = false;
+    Rect rc{0, 0, 0, 0};
+    uint32_t vi = 0;
+    if (active) {
+        if constexpr (BATCH) vi = find_le(ka.face_bases, ka.nviews, g);
+        const ViewDesc& V = view_at<BATCH>(ka, vi);
+        const uint32_t s = g - V.face_base;
+        const DevGroup G = group_of(V, s);
+        const uint32_t face = G.first + (s - G.rank_base);
+        const uint32_t* ip = V.idx + 3 * (size_t)face;
+        const uint32_t i0 = ip[0], i1 = ip[1], i2 = ip[2];
+        const vec4 c0 = V.mvp * make4(load_pos(V.pos, i0), 1.0f);
+        const vec4 c1 = V.mvp * make4(load_pos(V.pos, i1), 1.0f);
+        const vec4 c2 = V.mvp * make4(load_pos(V.pos, i2), 1.0f);
+        const int cls = classify(c0, c1, c2);
+        want_clip = cls == CLS_CLIP;
+        if (cls == CLS_INSIDE) {
+            TriRec r = make_rec(c0, c1, c2, V.W, V.H);
+            if (!is_backface(r)) {
+                RasterTri rt;
+                if (raster_setup(r, rt)) {
+                    rc = screen_rect(rt, V.W, V.H);
+                    if (rc.x0 < rc.x1 && rc.y0 < rc.y1) {
+                        r.c.y = s * 8u;
+                        r.c.z = face;
+                        r.c.w = G.material & REC_MATERIAL_MASK;
+                        ar.rec[g] = r;
+                        binnable = true;
+                    }
+                }
+            }
+        }
+    }
+    const uint32_t cslot = wave_append(&ar.counters[CNT_CLIP_FACES], want_clip);
+    if (want_clip) {
+        if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g;
+        else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
+    }
+    const uint32_t bslot = wave_append(&ar.counters[CNT_BINNABLE], binnable);
+    if (binnable) {
+        ar.binnable[bslot] = g;   // the face-slot array is sized so this fits
+        count_bins<T>(ar, view_at<BATCH>(ka, vi), rc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_clip: the reference's clipper on the faces k_setup queued.  A clipped
+// vertex is the 14 vertex floats followed by the clip position.
+
+struct CV { float f[18]; };
+
+__device__ __forceinline__ vec4 cv_clip(const CV& v) { return make4(v.f[14], v.f[15], v.f[16], v.f[17]); }
+
+__device__ void cv_interp(const CV& a, const CV& b, float t, CV& o) {  // pipeline.cpp:243-272
+    const float s = 1.0f - t;
+    for (int i = 0; i < 3; ++i) o.f[i] = a.f[i] * s + b.f[i] * t;   // position
+    for (int k = 5; k < 14; k += 3) {                                 // normal, tangent, bitangent
+        vec3 m = make3(a.f[k] * s + b.f[k] * t, a.f[k + 1] * s + b.f[k + 1] * t, a.f[k + 2] * s + b.f[k + 2] * t);
+        m = normalize(m);
+        o.f[k] = m.x; o.f[k + 1] = m.y; o.f[k + 2] = m.z;
+    }
+    for (int i = 14; i < 18; ++i) o.f[i] = a.f[i] * s + b.f[i] * t;  // clip position
+    const float w0 = a.f[17], w1 = b.f[17];
+    const float u = (a.f[3] * w0) * s + (b.f[3] * w1) * t;
+    const float v = (a.f[4] * w0) * s + (b.f[4] * w1) * t;
+    const float iw = w0 * s + w1 * t;
+    o.f[3] = u / iw;
+    o.f[4] = v / iw;
+}
+
+__device__ __forceinline__ float edge_t(float din, float dout) {
+    float denom = din - dout;
+    return (fabsf(denom) < 1e-7f) ? 0.5f : (din / denom);
+}
+
+__device__ __forceinline__ void cv_swap(CV& a, CV& b) { CV t = a; a = b; b = t; }
+
+// reference pipeline.cpp:302-370
+__device__ int clip_one_plane(int plane, CV& v0, CV& v1, CV& v2, CV& v3) {
+    float d0 = plane_dist(cv_clip(v0), plane), d1 = plane_dist(cv_clip(v1), plane), d2 = plane_dist(cv_clip(v2), plane);
+    if (d0 < 0.f && d1 < 0.f && d2 < 0.f) return 0;
+    if (d0 >= 0.f && d1 >= 0.f && d2 >= 0.f) { v3 = v0; return 3; }
+    const bool in0 = d0 >= 0.f, in1 = d1 >= 0.f, in2 = d2 >= 0.f;
+    if (in1 && !in0) {
+        cv_swap(v0, v1); float t = d0; d0 = d1; d1 = t;
+        cv_swap(v1, v2); t = d1; d1 = d2; d2 = t;
+    } else if (in2 && !in1) {
+        cv_swap(v2, v1); float t = d2; d2 = d1; d1 = t;
+        cv_swap(v1, v0); t = d1; d1 = d0; d0 = t;
+    }
+    cv_interp(v0, v2, edge_t(d0, d2), v3);
+    if (d1 >= 0.f) {
+        CV n2;
+        cv_interp(v1, v2, edge_t(d1, d2), n2);
+        v2 = n2;
+        return 4;
+    }
+    CV n1;
+    cv_interp(v0, v1, edge_t(d0, d1), n1);
+    v1 = n1;
+    v2 = v3;
+    return 3;
+}
+
+constexpr int MAX_CLIPPED_VERTS = 24;
+
+// reference pipeline.cpp:176-228; returns the vertex count (multiple of 3)
+__device__ int clip_triangle(CV* tris, CV* next) {
+    int count = 3;
+    for (int plane = 0; plane < 6; ++plane) {
+        int n = 0;
+        for (int i = 0; i < count; i += 3) {
+            CV v3;
+            int r = clip_one_plane(plane, tris[i], tris[i + 1], tris[i + 2], v3);
+            if (r == 3) {
+                if (n + 3 <= MAX_CLIPPED_VERTS) {
+                    next[n] = tris[i]; next[n + 1] = tris[i + 1]; next[n + 2] = tris[i + 2]; n += 3;
+                }
+            } else if (r == 4) {
+                if (n + 6 <= MAX_CLIPPED_VERTS) {
+                    next[n] = tris[i]; next[n + 1] = tris[i + 1]; next[n + 2] = tris[i + 2];
+                    next[n + 3] = tris[i]; next[n + 4] = tris[i + 2]; next[n + 5] = v3;
+                    n += 6;
+                }
+            }
+        }
+        count = n;
+        for (int i = 0; i < n; ++i) tris[i] = next[i];
+        if (count == 0) break;
+    }
+    return count;
+}
+
+template <int T, bool BATCH>
+__global__ __launch_bounds__(64) void k_clip(KArgs ka, Arena ar) {
+    if (ar.counters[CNT_ERROR]) return;
+    const uint32_t n = imin((int)ar.counters[CNT_CLIP_FACES], (int)ar.cap_clip_faces);
+    CV tris[MAX_CLIPPED_VERTS], next[MAX_CLIPPED_VERTS];
+    for (uint32_t q = blockIdx.x * 64u + threadIdx.x; q < n; q += gridDim.x * 64u) {
+        const uint32_t g = ar.clip_faces[q];
+        uint32_t vi = 0;
+        if constexpr (BATCH) vi = find_le(ka.face_bases, ka.nviews, g);
+        const ViewDesc& V = view_at<BATCH>(ka, vi);
+        const uint32_t s = g - V.face_base;
+        const DevGroup G = group_of(V, s);
+        const uint32_t face = G.first + (s - G.rank_base);
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t vtx = V.idx[3 * (size_t)face + k];
+            const float* src = reinterpret_cast<const float*>(V.verts + vtx);
+            for (int i = 0; i < 14; ++i) tris[k].f[i] = src[i];
+            vec4 c = V.mvp * make4(load_pos(V.pos, vtx), 1.0f);
+            tris[k].f[14] = c.x; tris[k].f[15] = c.y; tris[k].f[16] = c.z; tris[k].f[17] = c.w;
+        }
+        const int nv = clip_triangle(tris, next);
+        const uint32_t ntri = (uint32_t)(nv / 3);
+        if (ntri == 0) continue;
+        const uint32_t base = atomicAdd(&ar.counters[CNT_CLIP_RECS], ntri);
+        if (base + ntri > ar.cap_clip_recs) {
+            set_error(ar, ERR_CLIP_RECS, CNT_NEED_CLIP_RECS, base + ntri);
+            continue;
+        }
+        for (uint32_t j = 0; j < ntri; ++j) {
+            TriRec r = make_rec(cv_clip(tris[3 * j]), cv_clip(tris[3 * j + 1]), cv_clip(tris[3 * j + 2]), V.W, V.H);
+            if (is_backface(r)) continue;
+            RasterTri rt;
+            if (!raster_setup(r, rt)) continue;
+            Rect rc = screen_rect(rt, V.W, V.H);
+            if (!(rc.x0 < rc.x1 && rc.y0 < rc.y1)) continue;
+            const uint32_t slot = base + j;
+            r.c.y = s * 8u + j;
+            r.c.z = vi;
+            r.c.w = G.material & REC_MATERIAL_MASK;
+            ar.clip_rec[slot] = r;
+            for (int k = 0; k < 3; ++k) {
+                float* dst = reinterpret_cast<float*>(ar.clip_verts + 3 * (size_t)slot + k);
+                for (int i = 0; i < 14; ++i) dst[i] = tris[3 * j + k].f[i];
+            }
+            const uint32_t b = atomicAdd(&ar.counters[CNT_BINNABLE], 1u);
+            if (b < ar.cap_binnable) {
+                ar.binnable[b] = REF_CLIP | slot;
+                count_bins<T>(ar, V, rc);
+            } else {
+                set_error(ar, ERR_CLIP_RECS, CNT_NEED_CLIP_RECS, base + ntri);
+            }
+        }
+        TriRec m;
+        m.a = make_float4(0, 0, 0, 0);
+        m.b = make_float4(0, 0, 0, 0);
+        m.c = make_uint4(0u, s * 8u, base, REC_CLIPPED | (G.material & REC_MATERIAL_MASK));
+        ar.rec[g] = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// prefix sum of the bin counts: block reduce, top-level scan, down-sweep.
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = (int)__lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan across the 1024 threads of a block; returns the prefix and
+// the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+    const int lane = (int)__lane_id(), wave = threadIdx.x >> 6;
+    uint32_t incl = wave_incl_scan(v);
+    if (lane == 63) lds[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < (SCAN_THREADS / 64) ? lds[lane] : 0u;
+        uint32_t wi = wave_incl_scan(w);
+        if (lane < (SCAN_THREADS / 64)) lds[16 + lane] = wi - w;
+        if (lane == (SCAN_THREADS / 64) - 1) lds[32] = wi;
+    }
+    __syncthreads();
+    total = lds[32];
+    uint32_t r = lds[16 + wave] + incl - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Arena ar, uint32_t ntiles) {
+    __shared__ uint32_t lds[64];
+    if (ar.counters[CNT_ERROR]) return;
+    const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) if (base + k < ntiles) s += ar.tile_count[base + k];
+    uint32_t total;
+    block_excl_scan(s, lds, total);
+    if (threadIdx.x == 0) ar.scan_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(Arena ar, uint32_t nblocks) {
+    __shared__ uint32_t lds[64];
+    __shared__ uint32_t carry;
+    if (ar.counters[CNT_ERROR]) return;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t start = 0; start < nblocks; start += SCAN_THREADS) {
+        const uint32_t i = start + threadIdx.x;
+        uint32_t v = i < nblocks ? ar.scan_sums[i] : 0u;
+        uint32_t total;
+        uint32_t ex = block_excl_scan(v, lds, total);
+        const uint32_t c = carry;
+        if (i < nblocks) ar.scan_sums[i] = c + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t pairs = carry;
+        ar.counters[CNT_PAIRS] = pairs;
+        if (pairs > ar.cap_pairs) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, pairs);
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Arena ar, uint32_t ntiles) {
+    __shared__ uint32_t lds[64];
+    if (ar.counters[CNT_ERROR]) return;
+    const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (base + k < ntiles) ? ar.tile_count[base + k] : 0u; s += v[k]; }
+    uint32_t total;
+    uint32_t ex = block_excl_scan(s, lds, total) + ar.scan_sums[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < ntiles) ar.tile_off[base + k] = ex;
+        ex += v[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_scatter: one lane per binnable record, one entry per bin it overlaps.
+
+template <int T, bool BATCH>
+__global__ __launch_bounds__(256) void k_scatter(KArgs ka, Arena ar) {
+    if (ar.counters[CNT_ERROR]) return;
+    const uint32_t n = imin((int)ar.counters[CNT_BINNABLE], (int)ar.cap_binnable);
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < n; q += gridDim.x * 256u) {
+        const uint32_t ref = ar.binnable[q];
+        TriRec r;
+        uint32_t vi = 0;
+        if (ref & REF_CLIP) {
+            r = ar.clip_rec[ref & ~REF_CLIP];
+            vi = r.c.z;
+        } else {
+            r = ar.rec[ref];
+            if constexpr (BATCH) vi = find_le(ka.face_bases, ka.nviews, ref);
+        }
+        const ViewDesc& V = view_at<BATCH>(ka, vi);
+        RasterTri rt;
+        raster_setup(r, rt);
+        const Rect rc = screen_rect(rt, V.W, V.H);
+        const int tx0 = rc.x0 / T, tx1 = (rc.x1 - 1) / T;
+        const int ty0 = rc.y0 / T, ty1 = (rc.y1 - 1) / T;
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const uint32_t b = V.tile_base + (uint32_t)(ty * V.tiles_x + tx);
+                const uint32_t p = atomicAdd(&ar.tile_cursor[b], 1u);
+                ar.bins[ar.tile_off[b] + p] = ref;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_raster: one workgroup per bin.
+
+// Pixels at or below this count are walked by one lane; bigger footprints
+// are spread over the whole workgroup.
+constexpr int SMALL_FOOTPRINT = 12;
+
+__device__ __forceinline__ const TriRec& rec_of(const Arena& ar, uint32_t ref) {
+    return (ref & REF_CLIP) ? ar.clip_rec[ref & ~REF_CLIP] : ar.rec[ref];
+}
+
+template <class S>
+__device__ __forceinline__ vec4 shade_winner(const ViewDesc& V, const Arena& ar, const TriRec& r, uint32_t slot,
+                                            bool clipped, uint32_t rank, vec3 bar) {
+    ShadeCtx c;
+    c.u = &V.u;
+    c.m = &V.materials[r.c.w & REC_MATERIAL_MASK];
+    c.face_colors = V.face_colors;
+    typename S::Var var[3];
+    if (clipped) {
+        const DevGroup G = group_of(V, rank);
+        c.prim = G.first + (rank - G.rank_base);
+        if constexpr (sizeof(typename S::Var) > 1) {
+            for (int k = 0; k < 3; ++k) var[k] = S::vs(ar.clip_verts[3 * (size_t)slot + k], V.u);
+        }
+    } else {
+        c.prim = r.c.z;
+        if constexpr (sizeof(typename S::Var) > 1) {
+            const uint32_t* ip = V.idx + 3 * (size_t)r.c.z;
+            for (int k = 0; k < 3; ++k) var[k] = S::vs(V.verts[ip[k]], V.u);
+        }
+    }
+    return S::fs(bar, var, c);
+}
+
+template <class S>
+__device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px, int py) {
+    const uint32_t key = (uint32_t)k;
+    const uint32_t rank = key >> 3, j = key & 7u;
+    TriRec r = ar.rec[V.face_base + rank];
+    bool clipped = false;
+    uint32_t slot = 0;
+    if (r.c.w & REC_CLIPPED) {
+        clipped = true;
+        slot = r.c.z + j;
+        r = ar.clip_rec[slot];
+    }
+    RasterTri rt;
+    raster_setup(r, rt);
+    float al, be, ga, z;
+    cover(rt, px, py, al, be, ga, z);
+    const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
+    if (!(z < V.depth[pix])) return;   // reference mergeTileResults: strict less
+    const vec4 col = shade_winner<S>(V, ar, r, slot, clipped, rank, make3(al, be, ga));
+    V.depth[pix] = z;
+    V.color[pix] = pack_bgra(col);
+    if (V.ids) V.ids[pix] = key;
+}
+
+template <int T, int NT, class S>
+__device__ __forceinline__ void resolve_tile(const ViewDesc& V, const Arena& ar, const unsigned long long* keys,
+                                             int ox, int oy) {
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        const unsigned long long k = keys[p];
+        if (k == EMPTY_KEY) continue;
+        resolve_pixel<S>(V, ar, k, ox + (p % T), oy + (p / T));
+    }
+}
+
+template <int T, int NT, bool BATCH>
+__global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
+    __shared__ unsigned long long keys[T * T];
+    __shared__ uint32_t bigq[NT];
+    __shared__ uint32_t nbig;
+    if (ar.counters[CNT_ERROR]) return;
+    const uint32_t t = blockIdx.x;
+    uint32_t vi = 0;
+    if constexpr (BATCH) vi = find_le(ka.tile_bases, ka.nviews, t);
+    const ViewDesc& V = view_at<BATCH>(ka, vi);
+    const uint32_t n = ar.tile_count[t];
+    if (n == 0) return;
+    const uint32_t off = ar.tile_off[t];
+    const uint32_t local = t - V.tile_base;
+    const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
+    const int ex = imin(ox + T, V.W), ey = imin(oy + T, V.H);
+
+    for (int p = threadIdx.x; p < T * T; p += NT) keys[p] = EMPTY_KEY;
+    if (threadIdx.x == 0) nbig = 0;
+    __syncthreads();
+
+    for (uint32_t b0 = 0; b0 < n; b0 += NT) {
+        const uint32_t i = b0 + threadIdx.x;
+        if (i < n) {
+            const uint32_t ref = ar.bins[off + i];
+            const TriRec r = rec_of(ar, ref);
+            RasterTri rt;
+            if (raster_setup(r, rt)) {
+                const int x0 = imax(ox, rt.fx0), x1 = imin(ex, rt.cx1);
+                const int y0 = imax(oy, rt.fy0), y1 = imin(ey, rt.cy1);
+                const int w = x1 - x0, h = y1 - y0;
+                if (w > 0 && h > 0) {
+                    if (w * h <= SMALL_FOOTPRINT) {
+                        const uint32_t key = r.c.y;
+                        for (int py = y0; py < y1; ++py)
+                            for (int px = x0; px < x1; ++px) {
+                                float al, be, ga, z;
+                                if (cover(rt, px, py, al, be, ga, z) && z < INFINITY)
+                                    atomicMin(&keys[(py - oy) * T + (px - ox)], depth_key(z, key));
+                            }
+                    } else {
+                        bigq[atomicAdd(&nbig, 1u)] = ref;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t nb = nbig;
+        for (uint32_t q = 0; q < nb; ++q) {
+            const TriRec r = rec_of(ar, bigq[q]);
+            RasterTri rt;
+            raster_setup(r, rt);
+            const int x0 = imax(ox, rt.fx0), x1 = imin(ex, rt.cx1);
+            const int y0 = imax(oy, rt.fy0), y1 = imin(ey, rt.cy1);
+            const int w = x1 - x0, np = w * (y1 - y0);
+            const uint32_t key = r.c.y;
+            for (int p = threadIdx.x; p < np; p += NT) {
+                const int px = x0 + p % w, py = y0 + p / w;
+                float al, be, ga, z;
+                if (cover(rt, px, py, al, be, ga, z) && z < INFINITY)
+                    atomicMin(&keys[(py - oy) * T + (px - ox)], depth_key(z, key));
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) nbig = 0;
+        __syncthreads();
+    }
+
+    switch (V.u.kind) {
+    case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy); break;
+    case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy); break;
+    case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy); break;
+    case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy); break;
+    case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy); break;
+    default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// framebuffer clear (reference framebuffer.cpp:26-42)
+
+__global__ void k_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        color[i] = bgra;
+        depth[i] = d;
+        if (ids) ids[i] = 0xFFFFFFFFu;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch wrappers
+
+struct LaunchShape {
+    uint32_t setup_blocks, clip_blocks, scatter_blocks, scan_blocks;
+};
+
+template <int T, int NT, bool BATCH>
+static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+    const uint32_t ntiles = ka.total_tiles;
+    const uint32_t scan_blocks = (ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    k_prep<<<imax(1, imin(1024, (int)((ntiles + 255) / 256))), 256, 0, st>>>(ar, ntiles);
+    if (ka.total_faces) k_setup<T, BATCH><<<(ka.total_faces + 255) / 256, 256, 0, st>>>(ka, ar);
+    k_clip<T, BATCH><<<256, 64, 0, st>>>(ka, ar);
+    k_scan_reduce<<<imax(1, (int)scan_blocks), SCAN_THREADS, 0, st>>>(ar, ntiles);
+    k_scan_top<<<1, SCAN_THREADS, 0, st>>>(ar, scan_blocks);
+    k_scan_down<<<imax(1, (int)scan_blocks), SCAN_THREADS, 0, st>>>(ar, ntiles);
+    const uint32_t sblocks = imax(1, imin(4096, (int)((ka.total_faces + 255) / 256)));
+    k_scatter<T, BATCH><<<sblocks, 256, 0, st>>>(ka, ar);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    if (ntiles) k_raster<T, NT, BATCH><<<ntiles, NT, 0, st>>>(ka, ar);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+    const bool batch = ka.table != nullptr;
+    switch (ka.tile_size) {
+    case 8: return batch ? launch_frame_t<8, 64, true>(ka, ar, st, ev) : launch_frame_t<8, 64, false>(ka, ar, st, ev);
+    case 32: return batch ? launch_frame_t<32, 256, true>(ka, ar, st, ev) : launch_frame_t<32, 256, false>(ka, ar, st, ev);
+    default: return batch ? launch_frame_t<16, 256, true>(ka, ar, st, ev) : launch_frame_t<16, 256, false>(ka, ar, st, ev);
+    }
+}
+
+hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st) {
+    int blocks = (int)imin(2048, (int)((n + 255) / 256));
+    k_clear<<<imax(1, blocks), 256, 0, st>>>(color, depth, ids, n, bgra, d);
+    return hipGetLastError();
+}
+
+}  // namespace axr

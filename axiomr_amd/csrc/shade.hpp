@@ -1,0 +1,265 @@
+// shade.hpp - the fragment-stage shaders as device functors.
+//
+// Each shader is a struct with
+//   Var vs(const Vertex56&, const Uniforms&)         the vertex stage
+//   vec4 fs(vec3 bar, const Var v[3], const ShadeCtx&)  the fragment stage
+// and the raster kernel instantiates one resolve loop per shader, so the
+// shader choice costs one uniform branch per tile.
+//
+//   FlatShader        reference: include/shaders/shaders.hpp:19-62
+//   PhongShader       reference: include/shaders/shaders.hpp:140-247
+//   PBRShader         reference: include/shaders/shaders.hpp:252-423
+//   GouraudShader     no reference code: per-vertex Lambert intensity, times a
+//                     texture, colour = albedo * (0.1 + i)
+//   PhongPointShader  no reference code: PhongShader's normal-mapped model,
+//                     light terms summed over up to 4 point lights
+//   FlatColorShader   no reference code: per-face RGBA8, written back exactly
+// Texture::sample (reference include/texture.hpp:12-33) is restated as
+// sample_nearest; sample_bilinear is the added filter on the same texel grid.
+#pragma once
+#include "axr_types.hpp"
+
+#pragma clang fp contract(off)
+
+namespace axr {
+
+// float -> int32 with x86 cvttss2si semantics: truncation, NaN and
+// out-of-range give INT_MIN (the reference's (int) casts on its x86 target).
+AXR_HD int cvt_i32(float f) {
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : INT32_MIN;
+}
+
+AXR_HD vec4 texel(const DevTexture& t, int x, int y) {
+    uint32_t p = t.texels[(size_t)y * (size_t)t.w + (size_t)x];
+    const float inv255 = 1.0f / 255.0f;
+    return vec4{(float)(p & 255u) * inv255, (float)((p >> 8) & 255u) * inv255,
+                (float)((p >> 16) & 255u) * inv255, (float)(p >> 24) * inv255};
+}
+
+AXR_HD vec4 sample_nearest(const DevTexture& t, vec2 uv) {
+    int x = cvt_i32(uv.x * (float)(t.w - 1));
+    int y = cvt_i32(uv.y * (float)(t.h - 1));
+    x = imax(0, imin(x, t.w - 1));
+    y = imax(0, imin(y, t.h - 1));
+    return texel(t, x, t.h - 1 - y);
+}
+
+AXR_HD vec4 sample_bilinear(const DevTexture& t, vec2 uv) {
+    float sx = uv.x * (float)(t.w - 1);
+    float sy = uv.y * (float)(t.h - 1);
+    float flx = floorf(sx), fly = floorf(sy);
+    float fx = sx - flx, fy = sy - fly;
+    int x0 = cvt_i32(flx), y0 = cvt_i32(fly);
+    int x1 = (x0 == INT32_MAX) ? x0 : x0 + 1;
+    int y1 = (y0 == INT32_MAX) ? y0 : y0 + 1;
+    x0 = imax(0, imin(x0, t.w - 1)); x1 = imax(0, imin(x1, t.w - 1));
+    y0 = imax(0, imin(y0, t.h - 1)); y1 = imax(0, imin(y1, t.h - 1));
+    int r0 = t.h - 1 - y0, r1 = t.h - 1 - y1;
+    vec4 t00 = texel(t, x0, r0), t10 = texel(t, x1, r0);
+    vec4 t01 = texel(t, x0, r1), t11 = texel(t, x1, r1);
+    vec4 top = mix(t00, t10, fx);
+    vec4 bot = mix(t01, t11, fx);
+    return mix(top, bot, fy);
+}
+
+AXR_HD vec4 sample(const DevTexture& t, vec2 uv) {
+    if (!t.texels) return vec4{0.0f, 0.0f, 0.0f, 1.0f};
+    return t.filter ? sample_bilinear(t, uv) : sample_nearest(t, uv);
+}
+
+struct ShadeCtx {
+    const Uniforms* u;
+    const DevMaterial* m;
+    const uint32_t* face_colors;
+    uint32_t prim;     // mesh face index of the primitive
+};
+
+AXR_HD vec3 bary3(vec3 b, vec3 v0, vec3 v1, vec3 v2) { return (b.x * v0 + b.y * v1) + b.z * v2; }
+AXR_HD vec2 bary2(vec3 b, vec2 v0, vec2 v1, vec2 v2) { return (b.x * v0 + b.y * v1) + b.z * v2; }
+AXR_HD float bary1(vec3 b, float v0, float v1, float v2) { return (b.x * v0 + b.y * v1) + b.z * v2; }
+
+AXR_HD vec3 model_dir(const mat4& m, float x, float y, float z) { return xyz(m * make4(x, y, z, 0.0f)); }
+
+struct FlatShader {
+    struct Var { vec3 n; };
+    static AXR_HD Var vs(const Vertex56& v, const Uniforms& u) {
+        return Var{u.normal_mat * make3(v.nx, v.ny, v.nz)};
+    }
+    static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
+        vec3 n = normalize(bary3(bar, t[0].n, t[1].n, t[2].n));
+        float i = sclamp(dot(-c.u->light_dir, n), 0.0f, 1.0f);
+        return vec4{1.0f * i, 1.0f * i, 1.0f * i, 1.0f * i};
+    }
+};
+
+// Varyings of the normal-mapped shaders (reference shaders.hpp:147-168).
+struct TbnVar { vec2 uv; vec3 wp; vec3 t, b, n; };
+AXR_HD TbnVar tbn_vs(const Vertex56& v, const Uniforms& u) {
+    TbnVar o;
+    o.uv = make2(v.u, v.v);
+    o.wp = xyz(u.model * make4(v.px, v.py, v.pz, 1.0f));
+    o.t = normalize(model_dir(u.model, v.tx, v.ty, v.tz));
+    o.b = normalize(model_dir(u.model, v.bx, v.by, v.bz));
+    o.n = normalize(model_dir(u.model, v.nx, v.ny, v.nz));
+    return o;
+}
+// Interpolated TBN columns: (b.x*M0 + b.y*M1) + b.z*M2, column by column.
+AXR_HD void tbn_interp(vec3 bar, const TbnVar* t, vec3& iT, vec3& iB, vec3& iN) {
+    iT = (bar.x * t[0].t + bar.y * t[1].t) + bar.z * t[2].t;
+    iB = (bar.x * t[0].b + bar.y * t[1].b) + bar.z * t[2].b;
+    iN = (bar.x * t[0].n + bar.y * t[1].n) + bar.z * t[2].n;
+}
+// Normal-mapped world normal, albedo and position (reference shaders.hpp:172-218).
+AXR_HD void phong_surface(vec3 bar, const TbnVar* t, const DevMaterial& m, vec3& normal, vec4& albedo, vec3& fragPos) {
+    vec2 uv = bary2(bar, t[0].uv, t[1].uv, t[2].uv);
+    vec4 nmv = sample(m.bump, uv);
+    vec3 nms = normalize(xyz(nmv) * 2.0f - make3(1.0f, 1.0f, 1.0f));
+    vec3 iT, iB, iN;
+    tbn_interp(bar, t, iT, iB, iN);
+    vec3 N = normalize(iN);
+    vec3 T = normalize(iT - N * dot(N, iT));
+    vec3 B = cross(N, T);
+    mat3 f{{T, B, N}};
+    albedo = sample(m.diffuse, uv);
+    normal = normalize(f * nms);
+    fragPos = bary3(bar, t[0].wp, t[1].wp, t[2].wp);
+}
+
+struct PhongShader {
+    using Var = TbnVar;
+    static AXR_HD Var vs(const Vertex56& v, const Uniforms& u) { return tbn_vs(v, u); }
+    static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
+        vec3 normal, fragPos; vec4 albedo;
+        phong_surface(bar, t, *c.m, normal, albedo, fragPos);
+        const Uniforms& u = *c.u;
+        vec3 viewDir = normalize(u.camera_pos - fragPos);
+        vec3 lightDir = -u.light_dir;
+        vec3 ambient = 0.1f * u.light_color;
+        float diff = smax(dot(normal, lightDir), 0.0f);
+        vec3 diffuse = diff * u.light_color;
+        vec3 reflectDir = reflect(-lightDir, normal);
+        float spec = powf(smax(dot(viewDir, reflectDir), 0.0f), c.m->specular_exponent * 50.0f);
+        vec3 specular = (0.5f * spec) * u.light_color;
+        return make4(((ambient + diffuse) + specular) * xyz(albedo), 1.0f);
+    }
+};
+
+struct PhongPointShader {
+    using Var = TbnVar;
+    static AXR_HD Var vs(const Vertex56& v, const Uniforms& u) { return tbn_vs(v, u); }
+    static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
+        vec3 normal, fragPos; vec4 albedo;
+        phong_surface(bar, t, *c.m, normal, albedo, fragPos);
+        const Uniforms& u = *c.u;
+        vec3 viewDir = normalize(u.camera_pos - fragPos);
+        const float e = c.m->specular_exponent * 50.0f;
+        vec3 sum = make3(0.0f, 0.0f, 0.0f);
+        for (int l = 0; l < u.num_lights; ++l) {
+            vec3 L = normalize(u.light_pos[l] - fragPos);
+            vec3 ambient = 0.1f * u.light_colors[l];
+            float diff = smax(dot(normal, L), 0.0f);
+            vec3 diffuse = diff * u.light_colors[l];
+            vec3 reflectDir = reflect(-L, normal);
+            float spec = powf(smax(dot(viewDir, reflectDir), 0.0f), e);
+            vec3 specular = (0.5f * spec) * u.light_colors[l];
+            sum = sum + ((ambient + diffuse) + specular);
+        }
+        return make4(sum * xyz(albedo), 1.0f);
+    }
+};
+
+AXR_HD vec3 vpow(vec3 a, float e) { return make3(powf(a.x, e), powf(a.y, e), powf(a.z, e)); }
+
+struct PBRShader {
+    using Var = TbnVar;
+    static AXR_HD Var vs(const Vertex56& v, const Uniforms& u) { return tbn_vs(v, u); }
+    static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
+        const float PI = 3.14159265358979323846f;
+        const DevMaterial& m = *c.m;
+        const Uniforms& u = *c.u;
+        vec2 uv = bary2(bar, t[0].uv, t[1].uv, t[2].uv);
+        vec4 nmv = sample(m.bump, uv);
+        vec3 nms = normalize((xyz(nmv) * 2.0f) - 1.0f);
+        vec3 iT, iB, iN;
+        tbn_interp(bar, t, iT, iB, iN);
+        mat3 itbn{{iT, iB, iN}};
+        vec3 albedo = vpow(xyz(sample(m.diffuse, uv)), 2.2f);
+        float metallic = sample(m.metallic, uv).x;
+        float roughness = sample(m.roughness, uv).x;
+        float ao = sample(m.ao, uv).x;
+        float r2 = roughness * roughness;
+        float r4 = r2 * r2;
+        float omm = 1.0f - metallic;
+        // The reference re-orthogonalises the frame but lights with the
+        // interpolated one (shaders.hpp:316-343); only the latter matters.
+        vec3 normal = normalize(itbn * nms);
+        vec3 fragPos = bary3(bar, t[0].wp, t[1].wp, t[2].wp);
+        vec3 viewDir = normalize(u.camera_pos - fragPos);
+        vec3 lightDir = -u.light_dir;
+        vec3 halfway = normalize(lightDir + viewDir);
+        float NdotH = smax(dot(normal, halfway), 0.0f);
+        float NdotV = smax(dot(normal, viewDir), 0.0f);
+        float NdotL = smax(dot(normal, lightDir), 0.0f);
+        vec3 c04 = make3(0.04f, 0.04f, 0.04f);
+        vec3 F0 = c04 + metallic * (albedo - c04);
+        float NdotH2 = NdotH * NdotH;
+        float dp = NdotH2 * (r4 - 1.0f) + 1.0f;
+        float NDF = r4 / ((PI * dp) * dp);
+        float r = roughness + 1.0f;
+        float k = (r * r) / 8.0f;
+        float nvk = NdotV * (1.0f - k) + k;
+        float nlk = NdotL * (1.0f - k) + k;
+        float G = (NdotV / nvk) * (NdotL / nlk);
+        float cosT = smax(dot(halfway, normal), 0.0f);
+        float omc = 1.0f - cosT;
+        float term = omc * omc; term = term * term; term = term * omc;
+        vec3 one = make3(1.0f, 1.0f, 1.0f);
+        vec3 F = F0 + (one - F0) * term;
+        vec3 numerator = (F * NDF) * G;
+        float denom = ((4.0f * NdotV) * NdotL) + 0.0001f;
+        vec3 specular = numerator / denom;
+        vec3 kD = (one - F) * omm;
+        vec3 diffuse = (kD * albedo) * (1.0f / PI);
+        vec3 ambient = (make3(0.03f, 0.03f, 0.03f) * albedo) * ao;
+        vec3 fc = ambient + (((diffuse + specular) * u.light_color) * NdotL);
+        fc = fc / (fc + one);
+        fc = vpow(fc, 1.0f / 2.2f);
+        return make4(fc, 1.0f);
+    }
+};
+
+struct GouraudShader {
+    struct Var { float i; vec2 uv; };
+    static AXR_HD Var vs(const Vertex56& v, const Uniforms& u) {
+        vec3 N = normalize(model_dir(u.model, v.nx, v.ny, v.nz));
+        return Var{sclamp(dot(-u.light_dir, N), 0.0f, 1.0f), make2(v.u, v.v)};
+    }
+    static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
+        float i = bary1(bar, t[0].i, t[1].i, t[2].i);
+        vec2 uv = bary2(bar, t[0].uv, t[1].uv, t[2].uv);
+        vec4 albedo = sample(c.m->diffuse, uv);
+        return make4(xyz(albedo) * (0.1f + i), 1.0f);
+    }
+};
+
+struct FlatColorShader {
+    struct Var { };
+    static AXR_HD Var vs(const Vertex56&, const Uniforms&) { return Var{}; }
+    static AXR_HD vec4 fs(vec3, const Var*, const ShadeCtx& c) {
+        uint32_t k = c.face_colors ? c.face_colors[c.prim] : 0xFFFFFFFFu;
+        const float inv255 = 1.0f / 255.0f;
+        return vec4{((float)(k & 255u) + 0.5f) * inv255, ((float)((k >> 8) & 255u) + 0.5f) * inv255,
+                    ((float)((k >> 16) & 255u) + 0.5f) * inv255, ((float)(k >> 24) + 0.5f) * inv255};
+    }
+};
+
+// (uint8_t)(clamp(c, 0, 1) * 255) with truncation (reference
+// tiled_pipeline.cpp:577-580); NaN converts to 0.
+AXR_HD uint32_t to_u8(float c) { return (uint32_t)cvt_i32(sclamp(c, 0.0f, 1.0f) * 255.0f) & 255u; }
+// Colour in the framebuffer's byte order B, G, R, A (reference
+// tiled_pipeline.cpp:1171-1174).
+AXR_HD uint32_t pack_bgra(vec4 c) {
+    return to_u8(c.z) | (to_u8(c.y) << 8) | (to_u8(c.x) << 16) | (to_u8(c.w) << 24);
+}
+
+}  // namespace axr

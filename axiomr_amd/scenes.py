@@ -1,0 +1,165 @@
+"""Seeded synthetic workloads for the render path.
+
+Each builder returns a ``SceneData`` of plain numpy arrays (mesh, textures,
+camera, shader uniforms); the GPU path and the CPU checker consume the same
+arrays.  Geometry and textures come from the native generators in
+libaxiomr_amd.so (host code, no GPU needed).
+
+  uv_sphere_scene   800x600 UV sphere 100x100, Gouraud x 256^2 checker (bilinear)
+  icosphere_scene   1920x1080 displaced icosphere (~1M faces at n=224), Phong
+                    with a 2048^2 noise albedo + 2048^2 normal map, 3 point lights
+  soup_scene        1920x1080 triangle soup (4M), flat per-face colours, exact
+                    duplicate triangles for depth ties
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _native as N
+
+
+@dataclass
+class TextureData:
+    rgba: np.ndarray            # H x W x 4 uint8
+    filter: int                 # 0 nearest, 1 bilinear
+
+
+@dataclass
+class MaterialData:
+    diffuse: Optional[str] = None
+    metallic: Optional[str] = None
+    bump: Optional[str] = None
+    roughness: Optional[str] = None
+    ao: Optional[str] = None
+    specular_exponent: float = 1.0
+
+
+@dataclass
+class SceneData:
+    name: str
+    vertices: np.ndarray        # (N, 14) float32
+    faces: np.ndarray           # (F, 3) uint32
+    width: int
+    height: int
+    camera_pos: tuple
+    camera_target: tuple
+    fov: float
+    shader: dict                # {"kind": int, ...shader fields}
+    groups: list = field(default_factory=list)          # [(material, first, count)]
+    materials: list = field(default_factory=list)       # [MaterialData]
+    textures: dict = field(default_factory=dict)        # name -> TextureData
+    face_colors: Optional[np.ndarray] = None
+    model: np.ndarray = field(default_factory=lambda: np.eye(4, dtype=np.float32).reshape(16))
+
+    @property
+    def nfaces(self) -> int:
+        return int(self.faces.shape[0])
+
+    def checksum(self) -> str:
+        h = hashlib.sha256()
+        for a in (self.vertices, self.faces):
+            h.update(np.ascontiguousarray(a).tobytes())
+        for k in sorted(self.textures):
+            h.update(self.textures[k].rgba.tobytes())
+        if self.face_colors is not None:
+            h.update(self.face_colors.tobytes())
+        return h.hexdigest()[:16]
+
+
+def _chk(code):
+    N.check(code)
+
+
+def uv_sphere(stacks=100, slices=100, radius=1.0):
+    nv = (stacks + 1) * (slices + 1)
+    nf = 2 * stacks * slices
+    v = np.zeros((nv, 14), np.float32)
+    f = np.zeros((nf, 3), np.uint32)
+    _chk(N.lib().axr_gen_uv_sphere(stacks, slices, radius, N.fptr(v), N.u32ptr(f)))
+    return v, f
+
+
+def icosphere(n=224, amp=0.1, seed=1):
+    nv = 10 * n * n + 2
+    nf = 20 * n * n
+    v = np.zeros((nv, 14), np.float32)
+    f = np.zeros((nf, 3), np.uint32)
+    _chk(N.lib().axr_gen_icosphere(n, amp, seed, N.fptr(v), N.u32ptr(f)))
+    return v, f
+
+
+def soup(ntri, W, H, fov=60.0, median_area=2.0, sigma=1.5, dup_frac=0.1, seed=3):
+    v = np.zeros((3 * ntri, 14), np.float32)
+    f = np.zeros((ntri, 3), np.uint32)
+    c = np.zeros(ntri, np.uint32)
+    _chk(N.lib().axr_gen_soup(ntri, W, H, fov, median_area, sigma, dup_frac, seed, N.fptr(v), N.u32ptr(f),
+                              N.u32ptr(c)))
+    return v, f, c
+
+
+def checker(w=256, h=256, cell=8):
+    t = np.zeros((h, w, 4), np.uint8)
+    _chk(N.lib().axr_gen_checker(w, h, cell, N.u8ptr(t)))
+    return t
+
+
+def noise_texture(w=2048, h=2048, seed=5):
+    t = np.zeros((h, w, 4), np.uint8)
+    _chk(N.lib().axr_gen_noise_texture(w, h, seed, N.u8ptr(t)))
+    return t
+
+
+def normal_map(w=2048, h=2048, min_z=0.7, seed=6):
+    t = np.zeros((h, w, 4), np.uint8)
+    _chk(N.lib().axr_gen_normal_map(w, h, min_z, seed, N.u8ptr(t)))
+    return t
+
+
+def orbit_position(radius, yaw_deg, pitch_deg):
+    y, p = math.radians(yaw_deg), math.radians(pitch_deg)
+    return (radius * math.cos(p) * math.sin(y), radius * math.sin(p), radius * math.cos(p) * math.cos(y))
+
+
+POINT_LIGHTS = [((2.0, 2.5, 3.0), (0.70, 0.62, 0.55)),
+                ((-3.0, 1.0, 2.0), (0.30, 0.40, 0.65)),
+                ((0.5, -3.0, 2.5), (0.30, 0.30, 0.30))]
+
+
+def uv_sphere_scene(width=800, height=600, stacks=100, slices=100) -> SceneData:
+    """UV sphere, Gouraud diffuse x bilinear checker, one directional light."""
+    v, f = uv_sphere(stacks, slices)
+    return SceneData(
+        name=f"uv_sphere_{stacks}x{slices}_{width}x{height}", vertices=v, faces=f, width=width, height=height,
+        camera_pos=(0.0, 0.0, 3.0), camera_target=(0.0, 0.0, 0.0), fov=60.0,
+        shader={"kind": 3, "lightDirection": (0.0, -1.0, 0.0)},
+        groups=[(0, 0, f.shape[0])], materials=[MaterialData(diffuse="checker", specular_exponent=1.0)],
+        textures={"checker": TextureData(checker(256, 256, 8), 1)})
+
+
+def icosphere_scene(n=224, width=1920, height=1080, tex=2048, yaw=25.0, pitch=15.0, radius=3.0,
+                    seed=1) -> SceneData:
+    """Displaced icosphere, normal-mapped Phong with 3 point lights."""
+    v, f = icosphere(n, 0.1, seed)
+    return SceneData(
+        name=f"icosphere_n{n}_{width}x{height}", vertices=v, faces=f, width=width, height=height,
+        camera_pos=orbit_position(radius, yaw, pitch), camera_target=(0.0, 0.0, 0.0), fov=60.0,
+        shader={"kind": 4, "lights": POINT_LIGHTS},
+        groups=[(0, 0, f.shape[0])],
+        materials=[MaterialData(diffuse="albedo", bump="normal", specular_exponent=0.64)],
+        textures={"albedo": TextureData(noise_texture(tex, tex, 5), 1),
+                  "normal": TextureData(normal_map(tex, tex, 0.7, 6), 1)})
+
+
+def soup_scene(ntri=4_000_000, width=1920, height=1080, seed=3, dup_frac=0.1) -> SceneData:
+    """Screen-space triangle soup with flat per-face colours."""
+    v, f, c = soup(ntri, width, height, 60.0, 2.0, 1.5, dup_frac, seed)
+    return SceneData(
+        name=f"soup_{ntri}_{width}x{height}", vertices=v, faces=f, width=width, height=height,
+        camera_pos=(0.0, 0.0, 0.0), camera_target=(0.0, 0.0, -1.0), fov=60.0,
+        shader={"kind": 5}, groups=[(0, 0, f.shape[0])], materials=[MaterialData()], face_colors=c)

@@ -1,0 +1,195 @@
+/* axiomr.h - C ABI of the MI355X tiled render path.
+ *
+ * The boundary sits where the reference renderer calls its pipeline:
+ *   Pipeline::drawMesh(const glm::mat4& model, const Mesh&)
+ *       reference: include/pipeline.hpp:25, src/tiled_pipeline.cpp:143-322
+ * plus the resources drawMesh reads (Mesh, Material, Texture), the target it
+ * writes (Framebuffer) and the camera uniforms it copies into the shader.
+ * All signatures use plain pointers, sizes and POD structs; no exception
+ * crosses the ABI.  Every call returns an axr_status (0 = ok); the message of
+ * the last failure on a context is available from axr_last_error.
+ *
+ * Threading: a context is used by one host thread at a time.  Work is
+ * enqueued on the context's HIP stream; axr_draw is synchronous unless
+ * AXR_DRAW_ASYNC is set, in which case completion (and any arena overflow) is
+ * reported by axr_sync.
+ */
+#ifndef AXIOMR_H
+#define AXIOMR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXR_VERSION_MAJOR 0
+#define AXR_VERSION_MINOR 1
+
+typedef struct axr_ctx axr_ctx;
+typedef struct axr_texture axr_texture;
+typedef struct axr_mesh axr_mesh;
+typedef struct axr_fb axr_fb;
+typedef struct axr_batch axr_batch;
+typedef struct axr_server axr_server;
+
+typedef enum axr_status {
+    AXR_OK = 0,
+    AXR_ERR_INVALID = -1,    /* bad argument or handle */
+    AXR_ERR_OOM = -2,        /* device or host allocation failed */
+    AXR_ERR_HIP = -3,        /* a HIP runtime call failed */
+    AXR_ERR_OVERFLOW = -4,   /* frame arena too small; frames since the last sync were not drawn */
+    AXR_ERR_BUDGET = -5,     /* the request does not fit the HBM budget */
+    AXR_ERR_NO_DEVICE = -6,  /* no usable GPU */
+    AXR_ERR_NOT_READY = -7   /* a polled result is not complete yet */
+} axr_status;
+
+/* Shader kinds.  FLAT, PHONG and PBR restate the reference's FlatShader,
+ * PhongShader and PBRShader (reference: include/shaders/shaders.hpp:19-62,
+ * 140-247, 252-423).  GOURAUD, PHONG_POINT and FLAT_COLOR have no reference
+ * counterpart: per-vertex Lambert times a texture, normal-mapped Phong with
+ * point lights, and one RGBA8 colour per face. */
+typedef enum axr_shader_kind {
+    AXR_SHADER_FLAT = 0,
+    AXR_SHADER_PHONG = 1,
+    AXR_SHADER_PBR = 2,
+    AXR_SHADER_GOURAUD = 3,
+    AXR_SHADER_PHONG_POINT = 4,
+    AXR_SHADER_FLAT_COLOR = 5
+} axr_shader_kind;
+
+typedef enum axr_filter {
+    AXR_FILTER_NEAREST = 0,   /* reference Texture::sample (include/texture.hpp:12-33) */
+    AXR_FILTER_BILINEAR = 1
+} axr_filter;
+
+/* reference Material (include/mesh.hpp:20-34): texture handles may be null. */
+typedef struct axr_material {
+    axr_texture* diffuse;
+    axr_texture* metallic;
+    axr_texture* bump;
+    axr_texture* roughness;
+    axr_texture* ao;
+    float specular_exponent;
+} axr_material;
+
+/* reference MaterialGroup (include/mesh.hpp:35-39): faces
+ * [first_face, first_face + face_count) drawn with materials[material].
+ * Groups are drawn in array order. */
+typedef struct axr_group {
+    uint32_t material;
+    uint32_t first_face;
+    uint32_t face_count;
+} axr_group;
+
+#define AXR_DRAW_ASYNC 1u
+
+/* Uniforms of one drawMesh call (reference tiled_pipeline.cpp:148-155 and the
+ * shaders' public fields).  Matrices are column-major, 16 floats. */
+typedef struct axr_draw_params {
+    float model[16];
+    float view_proj[16];
+    float viewport[16];
+    float camera_pos[3];
+    int32_t shader;              /* axr_shader_kind */
+    float light_dir[3];          /* FLAT, PHONG, PBR, GOURAUD */
+    float light_color[3];        /* PHONG, PBR */
+    int32_t num_lights;          /* PHONG_POINT: up to 4 */
+    float light_pos[4][3];
+    float light_colors[4][3];
+    uint32_t flags;              /* AXR_DRAW_ASYNC */
+} axr_draw_params;
+
+/* Tunables of the binning pipeline (the autotuner writes these). */
+typedef struct axr_config {
+    int32_t tile_size;           /* GPU bin size: 8, 16 or 32 pixels */
+    int32_t reserved[7];
+} axr_config;
+
+/* Per-kernel timing collected with HIP events when profiling is on. */
+typedef struct axr_kernel_stats {
+    uint64_t frames;
+    double frame_ms_total;       /* whole frame (setup .. raster) */
+    double raster_ms_total;      /* k_raster only */
+} axr_kernel_stats;
+
+/* ---- context ---------------------------------------------------------- */
+const char* axr_version(void);
+int axr_device_count(int* count);
+int axr_ctx_create(int device, uint64_t hbm_budget_bytes, axr_ctx** out);
+int axr_ctx_destroy(axr_ctx* ctx);
+const char* axr_last_error(const axr_ctx* ctx);
+int axr_ctx_set_config(axr_ctx* ctx, const axr_config* cfg);
+int axr_ctx_get_config(const axr_ctx* ctx, axr_config* cfg);
+int axr_ctx_set_profiling(axr_ctx* ctx, int enable);
+int axr_ctx_get_stats(axr_ctx* ctx, axr_kernel_stats* out);
+int axr_ctx_memory(const axr_ctx* ctx, uint64_t* bytes_in_use, uint64_t* arena_bytes);
+int axr_sync(axr_ctx* ctx);
+
+/* ---- resources (reference Texture, Mesh, Framebuffer) ----------------- */
+/* rgba: w*h*4 bytes, row 0 first, as the reference stores stb_image output
+ * (src/texture.cpp:21-35). */
+int axr_texture_create(axr_ctx* ctx, const uint8_t* rgba, int w, int h, int filter, axr_texture** out);
+int axr_texture_destroy(axr_texture* tex);
+
+/* verts56: nverts x 14 floats laid out as the reference Vertex (position,
+ * uv, normal, tangent, bitangent; include/mesh.hpp:9-18).  idx: 3 per face.
+ * face_colors: optional RGBA8 per face (FLAT_COLOR shader), may be null. */
+int axr_mesh_create(axr_ctx* ctx, const float* verts56, uint64_t nverts, const uint32_t* idx, uint64_t nfaces,
+                    const axr_group* groups, uint32_t ngroups, const axr_material* materials, uint32_t nmaterials,
+                    const uint32_t* face_colors, axr_mesh** out);
+int axr_mesh_destroy(axr_mesh* mesh);
+
+/* BGRA8 colour + FP32 depth (reference Framebuffer, include/framebuffer.hpp).
+ * with_ids != 0 adds a per-pixel winner-key plane (debugging/parity). */
+int axr_fb_create(axr_ctx* ctx, int w, int h, int with_ids, axr_fb** out);
+int axr_fb_destroy(axr_fb* fb);
+/* reference Framebuffer::clearColor/clearDepth (src/framebuffer.cpp:26-42):
+ * bgra packed as a<<24 | r<<16 | g<<8 | b.  Enqueued, not synchronous. */
+int axr_fb_clear(axr_fb* fb, uint32_t bgra, float depth);
+/* Synchronous read-back; any pointer may be null. */
+int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids);
+
+/* ---- the hot path ------------------------------------------------------ */
+/* reference TiledPipeline::drawMesh (src/tiled_pipeline.cpp:143-322). */
+int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_params* params);
+
+/* ---- batched form: many views in one graph-captured launch sequence ---- */
+/* Each view i draws meshes[i] into fbs[i] with params[i]. */
+int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const axr_mesh* const* meshes,
+                     const axr_draw_params* params, axr_batch** out);
+int axr_batch_launch(axr_batch* batch);    /* enqueue (graph replay); completes at axr_sync */
+int axr_batch_destroy(axr_batch* batch);
+
+/* ---- synthetic scenes (host only; no device work) ---------------------- */
+/* UV sphere of radius r with `stacks` x `slices` quads: (stacks+1)*(slices+1)
+ * vertices, 2*stacks*slices faces. */
+int axr_gen_uv_sphere(int stacks, int slices, float radius, float* verts56, uint32_t* idx);
+/* Geodesic icosphere, frequency n: 10n^2+2 vertices, 20n^2 faces, radius
+ * 1 + amp * valueNoise4(p) (4 octaves), lattice hashed with `seed`. */
+int axr_gen_icosphere(int n, float amp, uint64_t seed, float* verts56, uint32_t* idx);
+/* Screen-space triangle soup for a camera at the origin looking down -z
+ * (fov_deg vertical, aspect W/H): centres uniform over the W x H screen,
+ * screen area log-normal (median_area pixels, sigma), one view depth per
+ * triangle uniform in [0.1, 100], counter-clockwise (front-facing) corners.
+ * A dup_frac share of the triangles repeats an earlier triangle's corners
+ * exactly (with its own colour) to force exact depth ties.  Writes 3*ntri
+ * vertices, ntri faces and ntri RGBA8 colours. */
+int axr_gen_soup(uint64_t ntri, int W, int H, float fov_deg, float median_area, float sigma, float dup_frac,
+                 uint64_t seed, float* verts56, uint32_t* idx, uint32_t* colors);
+int axr_gen_checker(int w, int h, int cell, uint8_t* rgba);
+int axr_gen_noise_texture(int w, int h, uint64_t seed, uint8_t* rgba);
+int axr_gen_normal_map(int w, int h, float min_z, uint64_t seed, uint8_t* rgba);
+/* reference Camera (src/camera.cpp:30-81): view, projection (RH, z in
+ * [-w, w]) and projection*view, column-major; plus the viewport matrix
+ * (camera.cpp:191-205). */
+int axr_camera_matrices(const float pos[3], const float target[3], float fov_deg, float aspect, float znear,
+                        float zfar, int vp_w, int vp_h, float* view16, float* proj16, float* view_proj16,
+                        float* viewport16);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXIOMR_H */
