@@ -59,7 +59,9 @@ class AxrConfig(C.Structure):
 
 
 class AxrKernelStats(C.Structure):
-    _fields_ = [("frames", C.c_uint64), ("frame_ms_total", C.c_double), ("raster_ms_total", C.c_double)]
+    _fields_ = [("frames", C.c_uint64), ("frame_ms_total", C.c_double), ("stage_ms_total", C.c_double * 5)]
+
+    STAGES = ("setup", "clip", "scan", "scatter", "raster")
 
 
 _P = C.c_void_p

@@ -111,14 +111,14 @@ struct alignas(16) TriRec {
 };
 constexpr uint32_t REC_CLIPPED = 1u << 31;
 constexpr uint32_t REC_MATERIAL_MASK = 0xFFFFu;
-constexpr uint32_t REF_CLIP = 1u << 31;     // bin entry / binnable ref names a clip record
+constexpr uint32_t REF_CLIP = 1u << 31;     // a bin entry with this bit names a clip record
 
 // Counter block at the head of the frame arena.
 enum Counter : uint32_t {
     CNT_CLIP_FACES = 0,   // faces queued for the full clipper
     CNT_CLIP_RECS = 1,    // clip records reserved
-    CNT_BINNABLE = 2,     // records that reach at least one bin
-    CNT_PAIRS = 3,        // (tile, record) pairs
+    CNT_OVF_PAIRS = 2,    // pairs placed in the overflow pool
+    CNT_UNUSED = 3,
     CNT_ERROR = 4,        // sticky error bits, see ERR_*
     CNT_NEED_CLIP_FACES = 5,
     CNT_NEED_CLIP_RECS = 6,
@@ -136,17 +136,22 @@ struct Arena {
     uint32_t* clip_faces;      // global face slots queued for clipping
     TriRec* clip_rec;          // clipped sub-triangles
     Vertex56* clip_verts;      // 3 per clip record
-    uint32_t* binnable;        // refs of records that reach a bin
     uint32_t* tile_count;      // per bin
     uint32_t* tile_off;        // exclusive prefix of tile_count (+1 entry)
-    uint32_t* tile_cursor;     // scatter cursors
+    uint4* pairs;              // (bin, slot in bin, record ref, 0) per overlap:
+                               // WAVE_PAIRS per setup wave, then the overflow pool
+    uint32_t* wave_pairs;      // pairs each setup wave wrote into its own region
     uint32_t* bins;            // refs, grouped by bin
     uint32_t* scan_sums;       // per-scan-block partial sums
     uint32_t cap_clip_faces;
     uint32_t cap_clip_recs;
-    uint32_t cap_binnable;
-    uint32_t cap_pairs;
+    uint32_t cap_waves;        // setup waves with a region
+    uint32_t cap_ovf;          // overflow pool entries (after the regions)
+    uint32_t cap_bins;
 };
+// Pair region of one setup wave: 2 per lane on average.  A wave that needs
+// more takes them from the overflow pool with a single atomic.
+constexpr uint32_t WAVE_PAIRS = 128;
 
 // Kernel arguments common to every stage.  `table` points to `nviews`
 // descriptors in device memory; when it is null the single descriptor `one`

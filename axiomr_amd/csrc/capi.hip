@@ -14,6 +14,7 @@
 namespace axr {
 hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev);
 hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st);
+hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st);
 }
 
 using namespace axr;
@@ -38,13 +39,14 @@ struct axr_ctx {
     std::string err;
     axr_config cfg{};
     // frame arena
-    DevBuf counters, rec, clip_faces, clip_rec, clip_verts, binnable, tile_count, tile_off, tile_cursor, bins, scan_sums;
+    DevBuf counters, rec, clip_faces, clip_rec, clip_verts, tile_count, tile_off, pairs, wave_pairs, bins, scan_sums;
+    bool arena_dirty = true;   // bin counts / counters need a k_prep before the next frame
     Arena arena{};
     uint64_t arena_gen = 0;
     uint32_t* host_counters = nullptr;   // pinned, CNT_COUNT words
     // profiling
     bool profiling = false;
-    hipEvent_t ev[kEventRing][3] = {};
+    hipEvent_t ev[kEventRing][6] = {};
     int ev_head = 0, ev_pending = 0;
     axr_kernel_stats stats{};
 };
@@ -166,37 +168,42 @@ int ensure_arena(axr_ctx* ctx, uint64_t faces, uint64_t tiles, uint64_t need_cli
     Arena& a = ctx->arena;
     uint64_t clip_faces = std::max<uint64_t>({4096, faces / 16, need_clip_faces, a.cap_clip_faces});
     uint64_t clip_recs = std::max<uint64_t>({2 * clip_faces, need_clip_recs, a.cap_clip_recs});
-    uint64_t binnable = faces + clip_recs;
-    uint64_t pairs = std::max<uint64_t>({2 * faces + 16 * tiles + 4096, need_pairs, a.cap_pairs});
+    // one pair region per setup wave (setup runs 4 waves per 256 faces)
+    uint64_t waves = std::max<uint64_t>(4 * ((faces + 255) / 256), a.cap_waves);
+    uint64_t ovf = std::max<uint64_t>({faces / 4 + 16 * tiles + 4096, need_pairs, a.cap_ovf});
+    uint64_t nbins = std::max<uint64_t>({2 * faces + 16 * tiles + 4096, need_pairs, a.cap_bins});
     uint64_t scan_blocks = (tiles + 4095) / 4096 + 1;
     int r;
+    const uint64_t gen_start = ctx->arena_gen;
     if ((r = grow(ctx, ctx->counters, CNT_COUNT * 4))) return r;
     if ((r = grow(ctx, ctx->rec, std::max<uint64_t>(faces, 1) * sizeof(TriRec)))) return r;
     if ((r = grow(ctx, ctx->clip_faces, clip_faces * 4))) return r;
     if ((r = grow(ctx, ctx->clip_rec, clip_recs * sizeof(TriRec)))) return r;
     if ((r = grow(ctx, ctx->clip_verts, clip_recs * 3 * sizeof(Vertex56)))) return r;
-    if ((r = grow(ctx, ctx->binnable, binnable * 4))) return r;
     if ((r = grow(ctx, ctx->tile_count, (tiles + 1) * 4))) return r;
     if ((r = grow(ctx, ctx->tile_off, (tiles + 1) * 4))) return r;
-    if ((r = grow(ctx, ctx->tile_cursor, (tiles + 1) * 4))) return r;
-    if ((r = grow(ctx, ctx->bins, pairs * 4))) return r;
+    if ((r = grow(ctx, ctx->pairs, (waves * WAVE_PAIRS + ovf) * 16))) return r;
+    if ((r = grow(ctx, ctx->wave_pairs, waves * 4))) return r;
+    if ((r = grow(ctx, ctx->bins, nbins * 4))) return r;
     if ((r = grow(ctx, ctx->scan_sums, scan_blocks * 4))) return r;
     a.counters = (uint32_t*)ctx->counters.p;
     a.rec = (TriRec*)ctx->rec.p;
     a.clip_faces = (uint32_t*)ctx->clip_faces.p;
     a.clip_rec = (TriRec*)ctx->clip_rec.p;
     a.clip_verts = (Vertex56*)ctx->clip_verts.p;
-    a.binnable = (uint32_t*)ctx->binnable.p;
     a.tile_count = (uint32_t*)ctx->tile_count.p;
     a.tile_off = (uint32_t*)ctx->tile_off.p;
-    a.tile_cursor = (uint32_t*)ctx->tile_cursor.p;
+    a.pairs = (uint4*)ctx->pairs.p;
+    a.wave_pairs = (uint32_t*)ctx->wave_pairs.p;
     a.bins = (uint32_t*)ctx->bins.p;
     a.scan_sums = (uint32_t*)ctx->scan_sums.p;
     a.cap_clip_faces = (uint32_t)std::min<uint64_t>(ctx->clip_faces.bytes / 4, 0xFFFFFFFFull);
     a.cap_clip_recs = (uint32_t)std::min<uint64_t>(
         {ctx->clip_rec.bytes / sizeof(TriRec), ctx->clip_verts.bytes / (3 * sizeof(Vertex56)), 0x7FFFFFFFull});
-    a.cap_binnable = (uint32_t)std::min<uint64_t>(ctx->binnable.bytes / 4, 0xFFFFFFFFull);
-    a.cap_pairs = (uint32_t)std::min<uint64_t>(ctx->bins.bytes / 4, 0xFFFFFFFFull);
+    a.cap_waves = (uint32_t)std::min<uint64_t>(ctx->wave_pairs.bytes / 4, 0x7FFFFFFFull / WAVE_PAIRS);
+    a.cap_ovf = (uint32_t)std::min<uint64_t>(ctx->pairs.bytes / 16 - (uint64_t)a.cap_waves * WAVE_PAIRS, 0x7FFFFFFFull);
+    a.cap_bins = (uint32_t)std::min<uint64_t>(ctx->bins.bytes / 4, 0x7FFFFFFFull);
+    if (ctx->arena_gen != gen_start || ctx->tile_count.bytes / 4 < tiles + 1) ctx->arena_dirty = true;
     return AXR_OK;
 }
 
@@ -218,18 +225,23 @@ int read_counters(axr_ctx* ctx) {
 void harvest_events(axr_ctx* ctx) {
     while (ctx->ev_pending > 0) {
         int slot = (ctx->ev_head - ctx->ev_pending + kEventRing) % kEventRing;
-        hipEventSynchronize(ctx->ev[slot][2]);
-        float f = 0, r = 0;
-        hipEventElapsedTime(&f, ctx->ev[slot][0], ctx->ev[slot][2]);
-        hipEventElapsedTime(&r, ctx->ev[slot][1], ctx->ev[slot][2]);
+        hipEvent_t* e = ctx->ev[slot];
+        (void)hipEventSynchronize(e[5]);
+        float total = 0;
+        (void)hipEventElapsedTime(&total, e[0], e[5]);
         ctx->stats.frames += 1;
-        ctx->stats.frame_ms_total += f;
-        ctx->stats.raster_ms_total += r;
+        ctx->stats.frame_ms_total += total;
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e[k], e[k + 1]);
+            ctx->stats.stage_ms_total[k] += ms;
+        }
         --ctx->ev_pending;
     }
 }
 
-// Events for one frame: [0] before the frame, [1] before k_raster, [2] after.
+// Events for one frame: [0] before the frame, [1..5] after setup, clip,
+// scan, scatter and raster.
 hipEvent_t* next_events(axr_ctx* ctx) {
     if (!ctx->profiling) return nullptr;
     if (ctx->ev_pending == kEventRing) harvest_events(ctx);
@@ -239,11 +251,22 @@ hipEvent_t* next_events(axr_ctx* ctx) {
     return ev;
 }
 
+// Zero the bin counts and per-frame counters if the arena is fresh or an
+// overflow left them dirty.  Never inside a captured graph.
+int prep_if_dirty(axr_ctx* ctx) {
+    if (!ctx->arena_dirty) return AXR_OK;
+    hipError_t e = launch_prep(ctx->arena, (uint32_t)(ctx->tile_count.bytes / 4), ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "arena reset");
+    ctx->arena_dirty = false;
+    return AXR_OK;
+}
+
 int launch(axr_ctx* ctx, const KArgs& ka) {
+    int r = prep_if_dirty(ctx);
+    if (r) return r;
     hipEvent_t* ev = next_events(ctx);
     if (ev) (void)hipEventRecord(ev[0], ctx->stream);
-    hipEvent_t inner[2] = {ev ? ev[1] : nullptr, ev ? ev[2] : nullptr};
-    hipError_t e = launch_frame(ka, ctx->arena, ctx->stream, ev ? inner : nullptr);
+    hipError_t e = launch_frame(ka, ctx->arena, ctx->stream, ev ? ev + 1 : nullptr);
     return e == hipSuccess ? AXR_OK : hip_fail(ctx, e, "kernel launch");
 }
 
@@ -256,6 +279,7 @@ int check_overflow(axr_ctx* ctx, uint64_t faces, uint64_t tiles) {
     int r = ensure_arena(ctx, faces, tiles, nf ? nf + nf / 2 : 0, nr ? nr + nr / 2 : 0, np ? np + np / 4 : 0);
     if (r) return r;
     if ((r = reset_counters(ctx))) return r;
+    ctx->arena_dirty = true;
     return fail(ctx, AXR_ERR_OVERFLOW, "frame arena overflow (clip faces %llu, clip records %llu, bin pairs %llu); "
                 "arena grown, frames since the last sync were not drawn",
                 (unsigned long long)nf, (unsigned long long)nr, (unsigned long long)np);
@@ -349,8 +373,8 @@ int axr_ctx_destroy(axr_ctx* ctx) {
     if (!ctx) return AXR_ERR_INVALID;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->counters, &ctx->rec, &ctx->clip_faces, &ctx->clip_rec, &ctx->clip_verts, &ctx->binnable,
-                      &ctx->tile_count, &ctx->tile_off, &ctx->tile_cursor, &ctx->bins, &ctx->scan_sums};
+    DevBuf* bufs[] = {&ctx->counters, &ctx->rec, &ctx->clip_faces, &ctx->clip_rec, &ctx->clip_verts,
+                      &ctx->tile_count, &ctx->tile_off, &ctx->pairs, &ctx->wave_pairs, &ctx->bins, &ctx->scan_sums};
     for (DevBuf* b : bufs) free_buf(ctx, *b);
     for (auto& ring : ctx->ev)
         for (auto& e : ring)
@@ -699,6 +723,7 @@ int axr_batch_launch(axr_batch* b) {
     int r = ensure_arena(ctx, b->ka.total_faces, b->ka.total_tiles);
     if (r) return r;
     if (ctx->profiling) return launch(ctx, b->ka);   // events need the eager path
+    if ((r = prep_if_dirty(ctx))) return r;
     if (!b->exec || b->captured_gen != ctx->arena_gen || b->captured_tile != b->ka.tile_size) {
         if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
         if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }

@@ -224,25 +224,105 @@ constexpr unsigned long long EMPTY_KEY = ~0ull;
 // k_prep: zero per-frame counters (the sticky error word and the "need"
 // high-water marks are left alone) and the bin counters/cursors.
 
+// The arena cleans itself during a frame (k_raster zeroes the bin counts it
+// consumed and the per-frame counters), so k_prep runs only after the arena
+// is (re)allocated or after an overflow.
 __global__ void k_prep(Arena ar, uint32_t ntiles) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid < CNT_ERROR) ar.counters[tid] = 0;
     for (uint32_t i = tid; i < ntiles; i += gridDim.x * blockDim.x) {
         ar.tile_count[i] = 0;
-        ar.tile_cursor[i] = 0;
     }
 }
 
 // ---------------------------------------------------------------------------
 // binning helpers
 
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = (int)__lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Bin range of a pixel rectangle for bins of T x T pixels.
 template <int T>
-__device__ __forceinline__ void count_bins(const Arena& ar, const ViewDesc& V, const Rect& rc) {
-    const int tx0 = rc.x0 / T, tx1 = (rc.x1 - 1) / T;
-    const int ty0 = rc.y0 / T, ty1 = (rc.y1 - 1) / T;
-    for (int ty = ty0; ty <= ty1; ++ty)
-        for (int tx = tx0; tx <= tx1; ++tx)
-            atomicAdd(&ar.tile_count[V.tile_base + (uint32_t)(ty * V.tiles_x + tx)], 1u);
+struct BinRange {
+    int tx0, ty0, nx, ny;
+    __device__ __forceinline__ BinRange(const Rect& rc, bool on) {
+        if (on) {
+            tx0 = rc.x0 / T; ty0 = rc.y0 / T;
+            nx = (rc.x1 - 1) / T - tx0 + 1; ny = (rc.y1 - 1) / T - ty0 + 1;
+        } else {
+            tx0 = ty0 = nx = ny = 0;
+        }
+    }
+    __device__ __forceinline__ uint32_t count() const { return (uint32_t)(nx * ny); }
+    __device__ __forceinline__ uint32_t bin(uint32_t k, const ViewDesc& V) const {
+        return V.tile_base + (uint32_t)((ty0 + (int)(k / (uint32_t)nx)) * V.tiles_x + tx0 + (int)(k % (uint32_t)nx));
+    }
+};
+
+// Emit the (bin, slot, ref) pairs of one record per lane, for a whole wave
+// (all 64 lanes must call it).  The wave writes into its own region of the
+// pair array (no shared counter); lanes that hit the same bin in the same
+// round share one atomic on that bin's counter, which also hands every pair
+// its final slot inside the bin, so the scatter pass needs no atomics.
+template <int T>
+__device__ __forceinline__ void emit_pairs_wave(const Arena& ar, const ViewDesc& V, bool on, const Rect& rc,
+                                                uint32_t ref, uint32_t wave) {
+    const BinRange<T> br(rc, on);
+    const uint32_t n = br.count();
+    const uint32_t incl = wave_incl_scan(n);
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t base = wave * WAVE_PAIRS;
+    bool fits = true;
+    if (total > WAVE_PAIRS) {
+        uint32_t o = 0;
+        if (__lane_id() == 0) o = atomicAdd(&ar.counters[CNT_OVF_PAIRS], total);
+        o = __shfl(o, 0);
+        fits = o + total <= ar.cap_ovf;
+        if (!fits && __lane_id() == 0) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, o + total);
+        base = ar.cap_waves * WAVE_PAIRS + o;
+    }
+    if (__lane_id() == 0) ar.wave_pairs[wave] = total > WAVE_PAIRS ? 0u : total;
+    if (total == 0) return;
+    const uint32_t p0 = base + incl - n;
+    for (uint32_t k = 0; __ballot(k < n); ++k) {
+        const bool act = k < n;
+        const uint32_t bin = act ? br.bin(k, V) : 0u;
+        unsigned long long pending = __ballot(act);
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const uint32_t lb = __shfl(bin, leader);
+            const unsigned long long m = __ballot(act && bin == lb) & pending;
+            uint32_t slot = 0;
+            if ((int)__lane_id() == leader) slot = atomicAdd(&ar.tile_count[lb], (uint32_t)__popcll(m));
+            slot = __shfl(slot, leader);
+            if (((m >> __lane_id()) & 1ull) && fits)
+                ar.pairs[p0 + k] = make_uint4(lb, slot + (uint32_t)__popcll(m & __lanemask_lt()), ref, 0u);
+            pending &= ~m;
+        }
+    }
+}
+
+// Same for one lane on its own (divergent code such as the clipper).
+// The pairs go to the overflow pool.
+template <int T>
+__device__ __forceinline__ void emit_pairs_lane(const Arena& ar, const ViewDesc& V, const Rect& rc, uint32_t ref) {
+    const BinRange<T> br(rc, true);
+    const uint32_t n = br.count();
+    const uint32_t o = atomicAdd(&ar.counters[CNT_OVF_PAIRS], n);
+    const bool fits = o + n <= ar.cap_ovf;
+    if (!fits) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, o + n);
+    const uint32_t p0 = ar.cap_waves * WAVE_PAIRS + o;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t bin = br.bin(k, V);
+        const uint32_t slot = atomicAdd(&ar.tile_count[bin], 1u);
+        if (fits) ar.pairs[p0 + k] = make_uint4(bin, slot, ref, 0u);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -291,11 +371,7 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
         if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g;
         else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
     }
-    const uint32_t bslot = wave_append(&ar.counters[CNT_BINNABLE], binnable);
-    if (binnable) {
-        ar.binnable[bslot] = g;   // the face-slot array is sized so this fits
-        count_bins<T>(ar, view_at<BATCH>(ka, vi), rc);
-    }
+    emit_pairs_wave<T>(ar, view_at<BATCH>(ka, vi), binnable, rc, g, g >> 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -430,13 +506,7 @@ __global__ __launch_bounds__(64) void k_clip(KArgs ka, Arena ar) {
                 float* dst = reinterpret_cast<float*>(ar.clip_verts + 3 * (size_t)slot + k);
                 for (int i = 0; i < 14; ++i) dst[i] = tris[3 * j + k].f[i];
             }
-            const uint32_t b = atomicAdd(&ar.counters[CNT_BINNABLE], 1u);
-            if (b < ar.cap_binnable) {
-                ar.binnable[b] = REF_CLIP | slot;
-                count_bins<T>(ar, V, rc);
-            } else {
-                set_error(ar, ERR_CLIP_RECS, CNT_NEED_CLIP_RECS, base + ntri);
-            }
+            emit_pairs_lane<T>(ar, V, rc, REF_CLIP | slot);
         }
         TriRec m;
         m.a = make_float4(0, 0, 0, 0);
@@ -448,15 +518,6 @@ __global__ __launch_bounds__(64) void k_clip(KArgs ka, Arena ar) {
 
 // ---------------------------------------------------------------------------
 // prefix sum of the bin counts: block reduce, top-level scan, down-sweep.
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const int lane = (int)__lane_id();
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
 
 // Exclusive scan across the 1024 threads of a block; returns the prefix and
 // the block total.
@@ -476,6 +537,33 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
     uint32_t r = lds[16 + wave] + incl - v;
     __syncthreads();
     return r;
+}
+
+// One block scans all bin counts (used when there are at most
+// SMALL_SCAN_TILES bins); also checks the bins capacity.
+constexpr uint32_t SMALL_SCAN_TILES = 65536;
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_single(Arena ar, uint32_t ntiles) {
+    __shared__ uint32_t lds[64];
+    __shared__ uint32_t carry;
+    if (ar.counters[CNT_ERROR]) return;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t start = 0; start < ntiles; start += SCAN_BLOCK) {
+        const uint32_t base = start + threadIdx.x * SCAN_ITEMS;
+        uint32_t v[SCAN_ITEMS];
+        uint32_t s = 0;
+        for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (base + k < ntiles) ? ar.tile_count[base + k] : 0u; s += v[k]; }
+        uint32_t total;
+        uint32_t ex = block_excl_scan(s, lds, total) + carry;
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            if (base + k < ntiles) ar.tile_off[base + k] = ex;
+            ex += v[k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && carry > ar.cap_bins) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, carry);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Arena ar, uint32_t ntiles) {
@@ -506,11 +594,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(Arena ar, uint32_t nb
         if (threadIdx.x == 0) carry = c + total;
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const uint32_t pairs = carry;
-        ar.counters[CNT_PAIRS] = pairs;
-        if (pairs > ar.cap_pairs) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, pairs);
-    }
+    if (threadIdx.x == 0 && carry > ar.cap_bins) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, carry);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Arena ar, uint32_t ntiles) {
@@ -529,35 +613,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Arena ar, uint32_t n
 }
 
 // ---------------------------------------------------------------------------
-// k_scatter: one lane per binnable record, one entry per bin it overlaps.
+// k_scatter: every pair already knows its slot inside its bin; place it.
+// First the per-wave regions, then the overflow pool.
 
-template <int T, bool BATCH>
-__global__ __launch_bounds__(256) void k_scatter(KArgs ka, Arena ar) {
+__global__ __launch_bounds__(256) void k_scatter(Arena ar, uint32_t nwaves) {
     if (ar.counters[CNT_ERROR]) return;
-    const uint32_t n = imin((int)ar.counters[CNT_BINNABLE], (int)ar.cap_binnable);
-    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < n; q += gridDim.x * 256u) {
-        const uint32_t ref = ar.binnable[q];
-        TriRec r;
-        uint32_t vi = 0;
-        if (ref & REF_CLIP) {
-            r = ar.clip_rec[ref & ~REF_CLIP];
-            vi = r.c.z;
-        } else {
-            r = ar.rec[ref];
-            if constexpr (BATCH) vi = find_le(ka.face_bases, ka.nviews, ref);
+    const uint32_t stride = gridDim.x * 256u;
+    const uint32_t nreg = nwaves * WAVE_PAIRS;
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < nreg; q += stride) {
+        if ((q % WAVE_PAIRS) < ar.wave_pairs[q / WAVE_PAIRS]) {
+            const uint4 p = ar.pairs[q];
+            ar.bins[ar.tile_off[p.x] + p.y] = p.z;
         }
-        const ViewDesc& V = view_at<BATCH>(ka, vi);
-        RasterTri rt;
-        raster_setup(r, rt);
-        const Rect rc = screen_rect(rt, V.W, V.H);
-        const int tx0 = rc.x0 / T, tx1 = (rc.x1 - 1) / T;
-        const int ty0 = rc.y0 / T, ty1 = (rc.y1 - 1) / T;
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                const uint32_t b = V.tile_base + (uint32_t)(ty * V.tiles_x + tx);
-                const uint32_t p = atomicAdd(&ar.tile_cursor[b], 1u);
-                ar.bins[ar.tile_off[b] + p] = ref;
-            }
+    }
+    const uint32_t novf = imin((int)ar.counters[CNT_OVF_PAIRS], (int)ar.cap_ovf);
+    const uint4* ovf = ar.pairs + (size_t)ar.cap_waves * WAVE_PAIRS;
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < novf; q += stride) {
+        const uint4 p = ovf[q];
+        ar.bins[ar.tile_off[p.x] + p.y] = p.z;
     }
 }
 
@@ -640,9 +713,12 @@ __global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
     uint32_t vi = 0;
     if constexpr (BATCH) vi = find_le(ka.tile_bases, ka.nviews, t);
     const ViewDesc& V = view_at<BATCH>(ka, vi);
+    if (t == 0 && threadIdx.x < CNT_ERROR) ar.counters[threadIdx.x] = 0;   // per-frame counters, consumed
     const uint32_t n = ar.tile_count[t];
     if (n == 0) return;
     const uint32_t off = ar.tile_off[t];
+    __syncthreads();
+    if (threadIdx.x == 0) ar.tile_count[t] = 0;   // consumed: leave the arena clean for the next frame
     const uint32_t local = t - V.tile_base;
     const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
     const int ex = imin(ox + T, V.W), ey = imin(oy + T, V.H);
@@ -726,21 +802,32 @@ struct LaunchShape {
     uint32_t setup_blocks, clip_blocks, scatter_blocks, scan_blocks;
 };
 
+// ev (optional): 5 events recorded after setup, clip, scan, scatter and
+// raster, on the stream the kernels run on.  Expects a clean arena (bin
+// counts and per-frame counters zero) and leaves it clean.
 template <int T, int NT, bool BATCH>
 static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
     const uint32_t ntiles = ka.total_tiles;
-    const uint32_t scan_blocks = (ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    k_prep<<<imax(1, imin(1024, (int)((ntiles + 255) / 256))), 256, 0, st>>>(ar, ntiles);
-    if (ka.total_faces) k_setup<T, BATCH><<<(ka.total_faces + 255) / 256, 256, 0, st>>>(ka, ar);
-    k_clip<T, BATCH><<<256, 64, 0, st>>>(ka, ar);
-    k_scan_reduce<<<imax(1, (int)scan_blocks), SCAN_THREADS, 0, st>>>(ar, ntiles);
-    k_scan_top<<<1, SCAN_THREADS, 0, st>>>(ar, scan_blocks);
-    k_scan_down<<<imax(1, (int)scan_blocks), SCAN_THREADS, 0, st>>>(ar, ntiles);
-    const uint32_t sblocks = imax(1, imin(4096, (int)((ka.total_faces + 255) / 256)));
-    k_scatter<T, BATCH><<<sblocks, 256, 0, st>>>(ka, ar);
+    const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
+    if (setup_blocks) k_setup<T, BATCH><<<setup_blocks, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[0], st);
-    if (ntiles) k_raster<T, NT, BATCH><<<ntiles, NT, 0, st>>>(ka, ar);
+    k_clip<T, BATCH><<<256, 64, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[1], st);
+    if (ntiles <= SMALL_SCAN_TILES) {
+        k_scan_single<<<1, SCAN_THREADS, 0, st>>>(ar, ntiles);
+    } else {
+        const uint32_t scan_blocks = (ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
+        k_scan_reduce<<<scan_blocks, SCAN_THREADS, 0, st>>>(ar, ntiles);
+        k_scan_top<<<1, SCAN_THREADS, 0, st>>>(ar, scan_blocks);
+        k_scan_down<<<scan_blocks, SCAN_THREADS, 0, st>>>(ar, ntiles);
+    }
+    if (ev) (void)hipEventRecord(ev[2], st);
+    const uint32_t nwaves = setup_blocks * 4;
+    const uint32_t sblocks = imax(64, imin(2048, (int)((nwaves * WAVE_PAIRS / 8 + 255) / 256)));
+    k_scatter<<<sblocks, 256, 0, st>>>(ar, nwaves);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    k_raster<T, NT, BATCH><<<imax(1, (int)ntiles), NT, 0, st>>>(ka, ar);
+    if (ev) (void)hipEventRecord(ev[4], st);
     return hipGetLastError();
 }
 
@@ -751,6 +838,11 @@ hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEve
     case 32: return batch ? launch_frame_t<32, 256, true>(ka, ar, st, ev) : launch_frame_t<32, 256, false>(ka, ar, st, ev);
     default: return batch ? launch_frame_t<16, 256, true>(ka, ar, st, ev) : launch_frame_t<16, 256, false>(ka, ar, st, ev);
     }
+}
+
+hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st) {
+    k_prep<<<imax(1, imin(1024, (int)((ntiles + 255) / 256))), 256, 0, st>>>(ar, ntiles);
+    return hipGetLastError();
 }
 
 hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st) {

@@ -163,3 +163,33 @@ def soup_scene(ntri=4_000_000, width=1920, height=1080, seed=3, dup_frac=0.1) ->
         name=f"soup_{ntri}_{width}x{height}", vertices=v, faces=f, width=width, height=height,
         camera_pos=(0.0, 0.0, 0.0), camera_target=(0.0, 0.0, -1.0), fov=60.0,
         shader={"kind": 5}, groups=[(0, 0, f.shape[0])], materials=[MaterialData()], face_colors=c)
+
+
+def with_shader(scene: SceneData, shader: dict, materials=None, textures=None, name=None) -> SceneData:
+    """Copy of `scene` drawn with another shader (and optionally materials)."""
+    import copy
+    s = copy.copy(scene)
+    s.shader = shader
+    if materials is not None:
+        s.materials = materials
+    if textures is not None:
+        s.textures = textures
+    s.name = name or f"{scene.name}_shader{shader['kind']}"
+    return s
+
+
+def pbr_scene(n=48, width=512, height=384, tex=256, seed=1) -> SceneData:
+    """Icosphere with the reference's PBR shader and five textures."""
+    v, f = icosphere(n, 0.1, seed)
+    return SceneData(
+        name=f"pbr_icosphere_n{n}_{width}x{height}", vertices=v, faces=f, width=width, height=height,
+        camera_pos=orbit_position(2.6, -30.0, 20.0), camera_target=(0.0, 0.0, 0.0), fov=60.0,
+        shader={"kind": 2, "lightDirection": (0.3, -1.0, -0.4), "lightColor": (5.6, 5.6, 5.6)},
+        groups=[(0, 0, f.shape[0])],
+        materials=[MaterialData(diffuse="albedo", metallic="metal", bump="normal", roughness="rough", ao="ao",
+                                specular_exponent=1.0)],
+        textures={"albedo": TextureData(noise_texture(tex, tex, 21), 0),
+                  "normal": TextureData(normal_map(tex, tex, 0.7, 22), 0),
+                  "metal": TextureData(noise_texture(tex, tex, 23), 1),
+                  "rough": TextureData(noise_texture(tex, tex, 24), 1),
+                  "ao": TextureData(noise_texture(tex, tex, 25), 0)})

@@ -107,11 +107,14 @@ typedef struct axr_config {
     int32_t reserved[7];
 } axr_config;
 
-/* Per-kernel timing collected with HIP events when profiling is on. */
+/* Stage timing collected with HIP events (recorded on the context's stream,
+ * between the kernels) when profiling is on.  Stages: 0 setup (counter reset,
+ * transform, classification, cull, triangle setup, bin counts), 1 clip,
+ * 2 prefix sum of the bin counts, 3 bin scatter, 4 per-bin raster + resolve. */
 typedef struct axr_kernel_stats {
     uint64_t frames;
-    double frame_ms_total;       /* whole frame (setup .. raster) */
-    double raster_ms_total;      /* k_raster only */
+    double frame_ms_total;
+    double stage_ms_total[5];
 } axr_kernel_stats;
 
 /* ---- context ---------------------------------------------------------- */

@@ -41,3 +41,66 @@ def test_soup_with_ties_and_clipping(ctx):
     sc = S.soup_scene(ntri=60_000, width=480, height=270, seed=9)
     rep = _run(ctx, sc)
     assert rep["covered"] > 50_000
+
+
+@pytest.mark.parametrize("kind", [0, 1, 3])
+def test_reference_shaders_on_icosphere(ctx, kind):
+    base = S.icosphere_scene(n=48, width=480, height=360, tex=256, yaw=-40.0)
+    shader = {0: {"kind": 0, "lightDirection": (0.2, -1.0, -0.3)},
+              1: {"kind": 1, "lightDirection": (0.2, -1.0, -0.3), "lightColor": (0.6, 0.6, 0.6)},
+              3: {"kind": 3, "lightDirection": (0.2, -1.0, -0.3)}}[kind]
+    sc = S.with_shader(base, shader)
+    _run(ctx, sc)
+
+
+def test_pbr_five_textures(ctx):
+    _run(ctx, S.pbr_scene())
+
+
+def test_flat_shader_rotated_model(ctx):
+    # the flat shader's normal matrix comes from inverse(model)
+    sc = S.with_shader(S.uv_sphere_scene(width=400, height=300), {"kind": 0, "lightDirection": (0.5, -0.7, -0.2)})
+    c, s_ = np.cos(0.7), np.sin(0.7)
+    m = np.array([[c, 0, -s_, 0], [0, 1.3, 0, 0], [s_, 0, c, 0], [0.1, -0.2, -0.3, 1]], np.float32).reshape(16)
+    sc.model = m
+    _run(ctx, sc)
+
+
+def test_two_draws_cross_draw_ties(ctx):
+    # the same mesh twice at the same place: the second draw loses every tie
+    # (reference mergeTileResults: strict less); then shifted copies overlap
+    sc = S.icosphere_scene(n=32, width=400, height=300, tex=128)
+    p = H.params_of(sc)
+    shifted = np.eye(4, dtype=np.float32)
+    shifted[3, :3] = (0.35, 0.1, -0.2)
+    q = H.params_of(sc, model=shifted.reshape(16))
+    _run(ctx, sc, draws=[p, p])
+    _run(ctx, sc, draws=[p, q, p])
+
+
+def test_material_groups_out_of_order(ctx):
+    # groups listed out of face order: submission order is group order
+    sc = S.icosphere_scene(n=32, width=400, height=300, tex=128)
+    nf = sc.nfaces
+    a, b = nf // 3, 2 * nf // 3
+    sc.groups = [(1, b, nf - b), (0, 0, a), (1, a, b - a)]
+    sc.materials = [sc.materials[0], S.MaterialData(diffuse="normal", bump="albedo", specular_exponent=0.3)]
+    _run(ctx, sc)
+
+
+def test_near_plane_clipping_close_camera(ctx):
+    # camera just outside the displaced surface: many faces cross the near
+    # plane and the frustum sides and go through the full clipper
+    sc = S.icosphere_scene(n=64, width=320, height=240, tex=128, radius=1.12, yaw=10.0, pitch=5.0)
+    rep = _run(ctx, sc)
+    assert rep["covered"] > 0
+
+
+def test_camera_inside_sphere(ctx):
+    sc = S.uv_sphere_scene(width=320, height=240)
+    sc.camera_pos = (0.0, 0.05, 0.2)
+    sc.camera_target = (0.3, 0.0, -1.0)
+    # inside the sphere the inner faces are back-facing; flip the winding so
+    # they are drawn
+    sc.faces = sc.faces[:, [0, 2, 1]].copy()
+    _run(ctx, sc)
