@@ -1,0 +1,155 @@
+"""The CPU checker's raster and depth resolve against the per-pixel brute
+force in rastercheck.py on seeded small scenes: every covered pixel, its
+depth bits and its winning primitive must agree exactly.  The scenes put
+triangles straight into clip space (identity matrices) so every corner case
+can be placed by hand: exact duplicates (submission-order ties), -0/+0
+depth ties, zero-area and near-zero-area triangles, partial 16-pixel tiles,
+pixel centres on edges and corners."""
+import numpy as np
+import pytest
+
+import harness as H
+import rastercheck as RC
+from axiomr_amd import render as R
+from axiomr_amd import scenes as S
+
+f32 = np.float32
+
+
+def ndc_scene(tris_xyz, W, H, colors=None):
+    """tris_xyz: (T, 3, 3) NDC corners (w = 1)."""
+    T = tris_xyz.shape[0]
+    v = np.zeros((3 * T, 14), f32)
+    v[:, :3] = tris_xyz.reshape(-1, 3)
+    v[:, 5:8] = (0, 0, 1)
+    f = np.arange(3 * T, dtype=np.uint32).reshape(T, 3)
+    if colors is None:
+        colors = (np.arange(T, dtype=np.uint32) * 2654435761 & 0xFFFFFFFF).astype(np.uint32)
+    return S.SceneData(name=f"ndc_{T}_{W}x{H}", vertices=v, faces=f, width=W, height=H, camera_pos=(0, 0, 0),
+                       camera_target=(0, 0, -1), fov=60.0, shader={"kind": 5}, groups=[(0, 0, T)],
+                       materials=[S.MaterialData()], face_colors=colors)
+
+
+def identity_params(model=None, view_proj=None):
+    p = R.N.AxrDrawParams()
+    eye = np.eye(4, dtype=f32).reshape(16)
+    p.model[:] = [float(x) for x in (eye if model is None else model)]
+    p.view_proj[:] = [float(x) for x in (eye if view_proj is None else view_proj)]
+    p.viewport[:] = [float(x) for x in eye]
+    p.shader = 5
+    return p
+
+
+def mat_mul(a16, b16):
+    """A * B, column c = ((A0 b.x + A1 b.y) + A2 b.z) + A3 b.w."""
+    A = np.asarray(a16, f32).reshape(4, 4)
+    B = np.asarray(b16, f32).reshape(4, 4)
+    out = np.zeros((4, 4), f32)
+    for c in range(4):
+        out[c] = ((A[0] * B[c, 0] + A[1] * B[c, 1]) + A[2] * B[c, 2]) + A[3] * B[c, 3]
+    return out.reshape(16)
+
+
+def transform(m16, xyz):
+    """clip = M * (p, 1) with M column-major, (c0 x + c1 y) + (c2 z + c3 w),
+    one float32 rounding per operation."""
+    c = np.asarray(m16, f32).reshape(4, 4)
+    x, y, z = xyz[..., 0:1], xyz[..., 1:2], xyz[..., 2:3]
+    return ((c[0] * x + c[1] * y) + (c[2] * z + c[3] * f32(1))).astype(f32)
+
+
+def check(scene, tris_xyz, model=None, view_proj=None):
+    W, Hh = scene.width, scene.height
+    eye = np.eye(4, dtype=f32).reshape(16)
+    mvp = mat_mul(eye if view_proj is None else view_proj, eye if model is None else model)
+    clip = transform(mvp, tris_xyz.astype(f32))
+    sx, sy, z = RC.screen_setup(clip, W, Hh)
+    keep = ~RC.backface(sx, sy)
+    keys = np.arange(len(tris_xyz), dtype=np.uint32) * 8
+    depth, ids = RC.resolve(sx[keep], sy[keep], z[keep], keys[keep], W, Hh)
+    bgra, odepth, oids = H.render_oracle(scene, draws=[identity_params(model, view_proj)])
+    assert (odepth.view(np.uint32) == depth.view(np.uint32)).all(), "depth bits differ"
+    assert (oids == ids).all(), "winners differ"
+    # flat colours written back exactly, BGRA order
+    cov = ids != RC.EMPTY
+    col = scene.face_colors[(ids[cov] >> 3)]
+    rgba = np.stack([col & 255, (col >> 8) & 255, (col >> 16) & 255, col >> 24], -1).astype(np.uint8)
+    assert (bgra[cov] == rgba[:, [2, 1, 0, 3]]).all()
+    assert (bgra[~cov] == [0, 0, 0, 255]).all()
+    return cov.sum()
+
+
+def random_tris(rng, T, scale, zr=(-0.9, 0.9)):
+    c = rng.uniform(-0.95, 0.95, (T, 1, 2))
+    off = rng.normal(size=(T, 3, 2)) * scale
+    xy = np.clip(c + off, -0.99, 0.99)
+    zz = rng.uniform(*zr, (T, 1, 1)) + rng.normal(size=(T, 3, 1)) * 0.05
+    return np.concatenate([xy, np.clip(zz, -0.99, 0.99)], axis=2).astype(f32)
+
+
+@pytest.mark.parametrize("W,Hh,T,scale,seed", [(64, 48, 60, 0.15, 1), (72, 40, 200, 0.05, 2), (37, 53, 120, 0.3, 3),
+                                               (160, 90, 400, 0.04, 4)])
+def test_brute_force_random(W, Hh, T, scale, seed):
+    rng = np.random.default_rng(seed)
+    tris = random_tris(rng, T, scale)
+    # both windings: half of them are culled
+    assert check(ndc_scene(tris, W, Hh), tris) > 0
+
+
+def test_exact_duplicates_first_wins():
+    rng = np.random.default_rng(11)
+    base = random_tris(rng, 40, 0.2)
+    tris = np.concatenate([base, base[::-1], base[:10]])
+    cov = check(ndc_scene(tris, 64, 64), tris)
+    assert cov > 0
+    _, _, ids = H.render_oracle(ndc_scene(tris, 64, 64), draws=[identity_params()])
+    hit = ids[ids != RC.EMPTY] >> 3
+    assert (hit < 40).all(), "a later exact duplicate never wins"
+
+
+def test_signed_zero_depth_ties():
+    # -0.0 entries in both matrices keep a -0.0 depth through
+    # mvp = viewProj * model and the transform for corners with x, y < 0 and
+    # z = -0.0 (every partial sum is -0).  Depths -0.0 and +0.0 tie under the
+    # strict-less test, so the earlier triangle keeps the pixel and its sign.
+    vp = np.eye(4, dtype=f32)
+    vp[0, 2] = vp[1, 2] = vp[3, 2] = -0.0
+    vp = vp.reshape(16)
+    model = np.eye(4, dtype=f32)
+    model[3, 2] = -0.0
+    model = model.reshape(16)
+    t = np.array([[[-0.9, -0.9, 0.0], [-0.05, -0.9, 0.0], [-0.5, -0.05, 0.0]]], f32)
+    neg = t.copy()
+    neg[..., 2] = -0.0
+    mvp = mat_mul(vp, model)
+    assert np.signbit(transform(mvp, neg)[..., 2]).all() and not np.signbit(transform(mvp, t)[..., 2]).any()
+    for tris in (np.concatenate([neg, t]), np.concatenate([t, neg])):
+        assert check(ndc_scene(tris, 48, 48), tris, model, vp) > 50
+        _, d, ids = H.render_oracle(ndc_scene(tris, 48, 48), draws=[identity_params(model, vp)])
+        cov = ids != RC.EMPTY
+        assert (ids[cov] == 0).all()
+        assert (np.signbit(d[cov]) == np.signbit(tris[0, 0, 2])).all()
+
+
+def test_degenerate_and_tiny_triangles():
+    tris = np.array([
+        [[-0.5, -0.5, 0.1], [0.5, 0.5, 0.1], [0.0, 0.0, 0.1]],            # collinear
+        [[0.1, 0.1, 0.2], [0.1, 0.1, 0.2], [0.1, 0.1, 0.2]],              # a point
+        [[-0.3, -0.3, 0.3], [-0.3 + 1e-7, -0.3, 0.3], [-0.3, -0.3 + 1e-7, 0.3]],   # below 1e-12 area
+        [[-0.9, -0.9, 0.5], [0.9, -0.9, 0.5], [0.0, 0.9, 0.5]],            # a big one behind
+    ], f32)
+    check(ndc_scene(tris, 64, 48), tris)
+
+
+def test_pixel_centres_on_edges_and_corners():
+    # corners on exact pixel centres of a 64 x 32 target: ndc = 2*(p+0.5)/W - 1
+    W, Hh = 64, 32
+    def ndc(px, py):
+        return [2.0 * (px + 0.5) / W - 1.0, 2.0 * (py + 0.5) / Hh - 1.0]
+    tris = np.array([
+        [ndc(2, 2) + [0.4], ndc(20, 2) + [0.4], ndc(2, 20) + [0.4]],
+        [ndc(20, 2) + [0.3], ndc(20, 20) + [0.3], ndc(2, 20) + [0.3]],     # shares the diagonal edge
+        [ndc(30, 5) + [0.2], ndc(47, 5) + [0.2], ndc(47, 22) + [0.2]],     # crosses the x=32 and x=40 anchors
+        [ndc(15, 15) + [0.1], ndc(17, 15) + [0.1], ndc(16, 17) + [0.1]],
+    ], f32)
+    assert check(ndc_scene(tris, W, Hh), tris) > 300
