@@ -58,9 +58,24 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_example()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+EXAMPLE_SRC = PKG / "cpp" / "example_drop_in.cpp"
+EXAMPLE_BIN = BUILD / "example_drop_in"
+
+
+def build_example() -> Path:
+    """The C++ drop-in example (plain g++ against include/ and the library)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", f"-I{PKG / 'cpp'}", str(EXAMPLE_SRC),
+           f"-L{PKG}", "-laxiomr_amd", "-Wl,-rpath,$ORIGIN/..", "-o", str(EXAMPLE_BIN)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return EXAMPLE_BIN
 
 
 if __name__ == "__main__":

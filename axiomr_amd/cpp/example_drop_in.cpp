@@ -1,0 +1,59 @@
+// example_drop_in.cpp - the reference renderer's frame loop (clear, drawMesh,
+// read back) written against axiomr.hpp instead of the reference's
+// TiledPipeline: an 800x600 UV sphere (100 x 100) with Gouraud shading over a
+// bilinear 256^2 checker, camera at (0, 0, 3).  Prints the covered pixel
+// count and an FNV-1a hash of the BGRA and depth planes.
+//
+//   g++ -std=c++17 -O2 -I include -I axiomr_amd/cpp axiomr_amd/cpp/example_drop_in.cpp \
+//       -L axiomr_amd -laxiomr_amd -Wl,-rpath,<dir of libaxiomr_amd.so> -o example_drop_in
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "axiomr.hpp"
+
+int main(int argc, char** argv) {
+    int frames = argc > 1 ? std::atoi(argv[1]) : 1;
+    const int W = 800, H = 600, stacks = 100, slices = 100;
+    std::vector<AR::Vertex> verts((stacks + 1) * (slices + 1));
+    std::vector<uint32_t> idx(6 * stacks * slices);
+    if (axr_gen_uv_sphere(stacks, slices, 1.0f, reinterpret_cast<float*>(verts.data()), idx.data()) != AXR_OK) return 2;
+    std::vector<uint8_t> checker(256 * 256 * 4);
+    axr_gen_checker(256, 256, 8, checker.data());
+    try {
+        AR::Context ctx(0);
+        AR::Texture tex(ctx, checker.data(), 256, 256, AR::Texture::Bilinear);
+        AR::Material mat;
+        mat.diffuseTexture = &tex;
+        AR::Mesh mesh(ctx, verts, idx, {{0, 0, idx.size() / 3}}, {mat});
+        AR::Framebuffer fb(ctx, W, H);
+        AR::Camera cam({0, 0, 3}, {0, 0, 0}, 60.0f, (float)W / H);
+        cam.setViewport(0, 0, W, H);
+        AR::GpuTiledPipeline pipe(&cam, &fb);
+        AR::GouraudShader shader;
+        shader.lightDirection = {0, -1, 0};
+        pipe.setShader(&shader);
+        for (int f = 0; f < frames; ++f) {
+            fb.clearColor(AR::Color{0, 0, 0, 255});
+            fb.clearDepth();
+            pipe.drawMesh(AR::identity(), mesh);
+        }
+        std::vector<uint8_t> bgra(W * H * 4);
+        std::vector<float> depth(W * H);
+        fb.read(bgra.data(), depth.data());
+        uint64_t h = 1469598103934665603ull;
+        auto mixb = [&](const void* p, size_t n) {
+            const uint8_t* b = static_cast<const uint8_t*>(p);
+            for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+        };
+        mixb(bgra.data(), bgra.size());
+        mixb(depth.data(), depth.size() * 4);
+        size_t covered = 0;
+        for (float d : depth) covered += std::isfinite(d);
+        std::printf("covered %zu hash %016llx\n", covered, (unsigned long long)h);
+    } catch (const AR::Error& e) {
+        std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
+        return 1;
+    }
+    return 0;
+}

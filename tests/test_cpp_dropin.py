@@ -1,0 +1,42 @@
+"""The C++ mirror of the reference interface (axiomr_amd/cpp/axiomr.hpp):
+the example frame loop builds with plain g++ against the C ABI, fails
+loudly without a GPU, and on the GPU draws exactly what the Python path
+draws for the same scene."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import harness as H
+from axiomr_amd import build as B
+from axiomr_amd import scenes as S
+
+
+def test_example_builds_and_needs_a_gpu():
+    exe = B.build_example()
+    import ctypes as C
+    from axiomr_amd import _native as N
+    n = C.c_int(0)
+    N.lib().axr_device_count(C.byref(n))
+    if n.value > 0:
+        pytest.skip("a GPU is present")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "error -6" in r.stderr
+
+
+@pytest.mark.gpu
+def test_example_matches_python_path(ctx):
+    exe = B.EXAMPLE_BIN
+    assert exe.exists(), "build the library first"
+    r = subprocess.run([str(exe), "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    covered, _, hexhash = r.stdout.split()[1:4]
+    bgra, depth, _ = H.render_gpu(ctx, S.uv_sphere_scene(), tile=16, with_ids=False)
+    assert int(covered) == int(np.isfinite(depth).sum())
+    # hash the way the example does (BGRA bytes, then depth bytes)
+    h = 1469598103934665603
+    data = np.concatenate([bgra.reshape(-1), depth.view(np.uint8).reshape(-1)])
+    for x in data.tobytes():
+        h ^= x
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    assert f"{h:016x}" == hexhash
