@@ -15,6 +15,8 @@ namespace axr {
 hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev);
 hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st);
 hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st);
+hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
+                              uint32_t bgra, float d, hipStream_t st);
 }
 
 using namespace axr;
@@ -87,12 +89,18 @@ struct axr_batch {
     ViewDesc* d_views = nullptr;
     uint32_t* d_face_bases = nullptr;
     uint32_t* d_tile_bases = nullptr;
+    uint32_t* d_pix_bases = nullptr;
+    uint32_t total_pixels = 0;
+    bool clear = false;            // clear every view's framebuffer before drawing
+    uint32_t clear_bgra = 0;
+    float clear_depth = 0.0f;
     KArgs ka{};
     size_t bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t captured_gen = ~0ull;
     int captured_tile = 0;
+    uint64_t clear_version = 0, captured_clear = ~0ull;
 };
 
 namespace {
@@ -672,8 +680,8 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     b->ctx = ctx;
     b->nviews = nviews;
     b->views.resize(nviews);
-    std::vector<uint32_t> fbase(nviews + 1), tbase(nviews + 1);
-    uint64_t faces = 0, tiles = 0;
+    std::vector<uint32_t> fbase(nviews + 1), tbase(nviews + 1), pbase(nviews + 1);
+    uint64_t faces = 0, tiles = 0, pixels = 0;
     const int tile = ctx->cfg.tile_size;
     for (uint32_t i = 0; i < nviews; ++i) {
         if (!fbs[i] || !meshes[i] || fbs[i]->ctx != ctx || meshes[i]->ctx != ctx || !shader_ok(meshes[i], &params[i], ctx)) {
@@ -685,22 +693,30 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
         v.tile_base = (uint32_t)tiles;
         fbase[i] = (uint32_t)faces;
         tbase[i] = (uint32_t)tiles;
+        pbase[i] = (uint32_t)pixels;
         faces += v.nfaces;
         tiles += (uint64_t)v.tiles_x * v.tiles_y;
-        if (faces > 0x7FFFFFFFull || tiles > 0x7FFFFFFFull) { delete b; return fail(ctx, AXR_ERR_INVALID, "batch too large"); }
+        pixels += (uint64_t)v.W * v.H;
+        if (faces > 0x7FFFFFFFull || tiles > 0x7FFFFFFFull || pixels > 0xFFFFFFFFull) {
+            delete b;
+            return fail(ctx, AXR_ERR_INVALID, "batch too large");
+        }
         b->views[i] = v;
     }
     fbase[nviews] = (uint32_t)faces;
     tbase[nviews] = (uint32_t)tiles;
+    pbase[nviews] = (uint32_t)pixels;
+    b->total_pixels = (uint32_t)pixels;
     int r;
     const size_t sz_v = nviews * sizeof(ViewDesc), sz_b = (nviews + 1) * 4;
     if ((r = dev_alloc(ctx, (void**)&b->d_views, sz_v)) || (r = dev_alloc(ctx, (void**)&b->d_face_bases, sz_b)) ||
-        (r = dev_alloc(ctx, (void**)&b->d_tile_bases, sz_b)) || (r = upload(ctx, b->d_views, b->views.data(), sz_v)) ||
-        (r = upload(ctx, b->d_face_bases, fbase.data(), sz_b)) || (r = upload(ctx, b->d_tile_bases, tbase.data(), sz_b))) {
+        (r = dev_alloc(ctx, (void**)&b->d_tile_bases, sz_b)) || (r = dev_alloc(ctx, (void**)&b->d_pix_bases, sz_b)) ||
+        (r = upload(ctx, b->d_views, b->views.data(), sz_v)) || (r = upload(ctx, b->d_face_bases, fbase.data(), sz_b)) ||
+        (r = upload(ctx, b->d_tile_bases, tbase.data(), sz_b)) || (r = upload(ctx, b->d_pix_bases, pbase.data(), sz_b))) {
         axr_batch_destroy(b);
         return r;
     }
-    b->bytes = sz_v + 2 * sz_b;
+    b->bytes = sz_v + 3 * sz_b;
     ctx->in_use += b->bytes;
     KArgs& ka = b->ka;
     ka.table = b->d_views;
@@ -722,14 +738,26 @@ int axr_batch_launch(axr_batch* b) {
     hipSetDevice(ctx->device);
     int r = ensure_arena(ctx, b->ka.total_faces, b->ka.total_tiles);
     if (r) return r;
-    if (ctx->profiling) return launch(ctx, b->ka);   // events need the eager path
+    if (ctx->profiling) {   // events need the eager path
+        if (b->clear) {
+            hipError_t e = launch_clear_views(b->d_views, b->d_pix_bases, b->nviews, b->total_pixels, b->clear_bgra,
+                                              b->clear_depth, ctx->stream);
+            if (e != hipSuccess) return hip_fail(ctx, e, "batch clear");
+        }
+        return launch(ctx, b->ka);
+    }
     if ((r = prep_if_dirty(ctx))) return r;
-    if (!b->exec || b->captured_gen != ctx->arena_gen || b->captured_tile != b->ka.tile_size) {
+    if (!b->exec || b->captured_gen != ctx->arena_gen || b->captured_tile != b->ka.tile_size ||
+        b->captured_clear != b->clear_version) {
         if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
         if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
         hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
-        hipError_t le = launch_frame(b->ka, ctx->arena, ctx->stream, nullptr);
+        hipError_t le = hipSuccess;
+        if (b->clear)
+            le = launch_clear_views(b->d_views, b->d_pix_bases, b->nviews, b->total_pixels, b->clear_bgra,
+                                    b->clear_depth, ctx->stream);
+        if (le == hipSuccess) le = launch_frame(b->ka, ctx->arena, ctx->stream, nullptr);
         e = hipStreamEndCapture(ctx->stream, &b->graph);
         if (le != hipSuccess) return hip_fail(ctx, le, "batch capture launch");
         if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
@@ -737,9 +765,19 @@ int axr_batch_launch(axr_batch* b) {
         if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphInstantiate");
         b->captured_gen = ctx->arena_gen;
         b->captured_tile = b->ka.tile_size;
+        b->captured_clear = b->clear_version;
     }
     hipError_t e = hipGraphLaunch(b->exec, ctx->stream);
     return e == hipSuccess ? AXR_OK : hip_fail(ctx, e, "hipGraphLaunch");
+}
+
+int axr_batch_set_clear(axr_batch* b, int enable, uint32_t bgra, float depth) {
+    if (!b) return AXR_ERR_INVALID;
+    b->clear = enable != 0;
+    b->clear_bgra = bgra;
+    b->clear_depth = depth;
+    ++b->clear_version;
+    return AXR_OK;
 }
 
 int axr_batch_destroy(axr_batch* b) {
@@ -752,6 +790,7 @@ int axr_batch_destroy(axr_batch* b) {
     if (b->d_views) hipFree(b->d_views);
     if (b->d_face_bases) hipFree(b->d_face_bases);
     if (b->d_tile_bases) hipFree(b->d_tile_bases);
+    if (b->d_pix_bases) hipFree(b->d_pix_bases);
     ctx->in_use -= b->bytes;
     delete b;
     return AXR_OK;

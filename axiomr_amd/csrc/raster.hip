@@ -795,6 +795,20 @@ __global__ void k_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, 
     }
 }
 
+// Clear of every view of a batch: one lane per pixel of the concatenated
+// framebuffers (views found through the pixel prefix).
+__global__ void k_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
+                              uint32_t bgra, float d) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const uint32_t v = find_le(pix_bases, nviews, i);
+        const ViewDesc& V = views[v];
+        const uint32_t p = i - pix_bases[v];
+        V.color[p] = bgra;
+        V.depth[p] = d;
+        if (V.ids) V.ids[p] = 0xFFFFFFFFu;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host-side launch wrappers
 
@@ -838,6 +852,13 @@ hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEve
     case 32: return batch ? launch_frame_t<32, 256, true>(ka, ar, st, ev) : launch_frame_t<32, 256, false>(ka, ar, st, ev);
     default: return batch ? launch_frame_t<16, 256, true>(ka, ar, st, ev) : launch_frame_t<16, 256, false>(ka, ar, st, ev);
     }
+}
+
+hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
+                              uint32_t bgra, float d, hipStream_t st) {
+    k_clear_views<<<imax(1, imin(4096, (int)((total + 255) / 256))), 256, 0, st>>>(views, pix_bases, nviews, total,
+                                                                                   bgra, d);
+    return hipGetLastError();
 }
 
 hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st) {

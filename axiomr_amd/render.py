@@ -385,6 +385,13 @@ class Batch:
         self.handle = h
         self._keep = (list(fbs), list(meshes))
 
+    def set_clear(self, color: Optional["Color"], depth: float = float("inf")) -> None:
+        """Clear every view's framebuffer at the start of each launch (None: don't)."""
+        if color is None:
+            self.ctx.check(self.ctx._lib.axr_batch_set_clear(self.handle, 0, 0, depth))
+        else:
+            self.ctx.check(self.ctx._lib.axr_batch_set_clear(self.handle, 1, color.packed_bgra(), depth))
+
     def launch(self) -> None:
         self.ctx.check(self.ctx._lib.axr_batch_launch(self.handle))
 

@@ -193,3 +193,65 @@ def pbr_scene(n=48, width=512, height=384, tex=256, seed=1) -> SceneData:
                   "metal": TextureData(noise_texture(tex, tex, 23), 1),
                   "rough": TextureData(noise_texture(tex, tex, 24), 1),
                   "ao": TextureData(noise_texture(tex, tex, 25), 0)})
+
+
+@dataclass
+class ViewSpec:
+    mesh: int
+    width: int
+    height: int
+    camera_pos: tuple
+    camera_target: tuple = (0.0, 0.0, 0.0)
+    fov: float = 60.0
+
+
+@dataclass
+class BatchWorkload:
+    """Many views over a set of resident meshes sharing one material."""
+    meshes: list                  # [(vertices (N,14), faces (F,3))]
+    textures: dict                # name -> TextureData
+    material: MaterialData
+    shader: dict
+    views: list                   # [ViewSpec]
+
+    def scene(self, i: int) -> SceneData:
+        """View i as a standalone SceneData (for solo renders and the checker)."""
+        vs = self.views[i]
+        v, f = self.meshes[vs.mesh]
+        return SceneData(name=f"view{i}_mesh{vs.mesh}_{vs.width}x{vs.height}", vertices=v, faces=f,
+                         width=vs.width, height=vs.height, camera_pos=vs.camera_pos, camera_target=vs.camera_target,
+                         fov=vs.fov, shader=self.shader, groups=[(0, 0, f.shape[0])], materials=[self.material],
+                         textures=self.textures)
+
+    @property
+    def total_faces(self) -> int:
+        return sum(self.meshes[v.mesh][1].shape[0] for v in self.views)
+
+    @property
+    def total_pixels(self) -> int:
+        return sum(v.width * v.height for v in self.views)
+
+
+def batch_workload(nviews=256, nmeshes=32, fmin=5_000, fmax=500_000, wmin=256, wmax=1920, hmin=256, hmax=1080,
+                   tex=2048, seed=17) -> BatchWorkload:
+    """Log-uniform mesh sizes (icosphere face counts 20 n^2) and log-uniform
+    resolutions, random orbit cameras, normal-mapped Phong with 3 point
+    lights over a noise albedo and a normal map."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for i in range(nmeshes):
+        F = math.exp(rng.uniform(math.log(fmin), math.log(fmax)))
+        n = max(2, int(round(math.sqrt(F / 20.0))))
+        meshes.append(icosphere(n, 0.1, 100 + i))
+    views = []
+    for _ in range(nviews):
+        t = rng.uniform(0.0, 1.0)
+        w = int(round(wmin * (wmax / wmin) ** t))
+        h = int(round(hmin * (hmax / hmin) ** t))
+        pos = orbit_position(rng.uniform(2.2, 4.0), rng.uniform(0.0, 360.0), rng.uniform(-60.0, 60.0))
+        views.append(ViewSpec(mesh=int(rng.integers(nmeshes)), width=w, height=h, camera_pos=pos))
+    textures = {"albedo": TextureData(noise_texture(tex, tex, 5), 1),
+                "normal": TextureData(normal_map(tex, tex, 0.7, 6), 1)}
+    return BatchWorkload(meshes=meshes, textures=textures,
+                         material=MaterialData(diffuse="albedo", bump="normal", specular_exponent=0.64),
+                         shader={"kind": 4, "lights": POINT_LIGHTS}, views=views)

@@ -106,6 +106,46 @@ def run_plain(scene, steps, warmup, tile):
                 arena_bytes=arena)
 
 
+def run_batch(wl, reps, tile):
+    """The batched form (256 views, 32 meshes) against the plain per-frame
+    path on the same views: frames (views) per second of each."""
+    import harness as H
+    import axiomr_amd as axr
+
+    ctx = axr.Context(0)
+    gb = H.GpuBatch(ctx, wl, with_ids=False, tile=tile)
+    n = len(wl.views)
+    # plain per-frame path: clear + drawMesh for every view, queued
+    for i in range(n):
+        gb.draw_solo(i, flags=1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for i in range(n):
+            gb.draw_solo(i, flags=1)
+    ctx.sync()
+    plain_s = (time.perf_counter() - t0) / reps
+    # batched: one graph launch draws (and clears) every view
+    gb.batch.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gb.batch.launch()
+    ctx.sync()
+    batch_s = (time.perf_counter() - t0) / reps
+    ctx.set_profiling(True)
+    gb.batch.launch()
+    ctx.sync()
+    st = ctx.stats()
+    ctx.set_profiling(False)
+    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(st.STAGES)}
+    gb.close()
+    ctx.close()
+    return dict(views=n, plain_views_per_s=n / plain_s, batch_views_per_s=n / batch_s, speedup=plain_s / batch_s,
+                batch_ms=batch_s * 1e3, plain_ms=plain_s * 1e3, stages_ms=stages,
+                faces=wl.total_faces, pixels=wl.total_pixels)
+
+
 def cpu_baseline(scene, budget_s=12.0, threads=16):
     """The CPU checker (restated reference tiled pipeline with its thread
     pool) on the same frame, repeated until ~budget_s of CPU work."""
@@ -143,6 +183,8 @@ def main():
     ap.add_argument("--n", type=int, default=224, help="icosphere frequency (20 n^2 faces)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
+    ap.add_argument("--batch-reps", type=int, default=5)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -174,6 +216,24 @@ def main():
     kms = stages[dominant]
     achieved = kbytes / (kms * 1e-3) / 1e9
     frame_bytes = algorithmic_bytes(nverts, nfaces, W, H, r["covered"], ntex)
+
+    batch = None
+    if not args.no_batch:
+        t = time.perf_counter()
+        wl = S.batch_workload()
+        log(f"[bench] batch workload: {len(wl.views)} views, {wl.total_faces} faces, {wl.total_pixels} pixels "
+            f"({time.perf_counter() - t:.1f}s)")
+        b = run_batch(wl, args.batch_reps, args.tile)
+        log(f"[bench] batch: plain {b['plain_views_per_s']:.0f} views/s, batched {b['batch_views_per_s']:.0f} views/s, "
+            f"x{b['speedup']:.2f}; stages {b['stages_ms']}")
+        batch = {"workload": f"{len(wl.views)} views of 32 displaced icospheres (5k-500k triangles, log-uniform), "
+                             "256x256..1920x1080 log-uniform, Phong + textures, one graph launch per batch",
+                 "views": b["views"], "triangles": b["faces"], "pixels": b["pixels"],
+                 "batched_views_per_s": round(b["batch_views_per_s"], 1),
+                 "plain_views_per_s": round(b["plain_views_per_s"], 1),
+                 "speedup_vs_plain": round(b["speedup"], 3),
+                 "batched_gtri_per_s": round(b["faces"] / (b["batch_ms"] * 1e-3) / 1e9, 3),
+                 "batch_ms": round(b["batch_ms"], 3), "stages_ms": {k: round(v, 4) for k, v in b["stages_ms"].items()}}
 
     cpu = None
     if not args.no_cpu:
@@ -208,6 +268,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 5)},
         "cpu_baseline": cpu,
+        "batched": batch,
     }
     print(json.dumps(out), flush=True)
 

@@ -162,6 +162,9 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
 /* Each view i draws meshes[i] into fbs[i] with params[i]. */
 int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const axr_mesh* const* meshes,
                      const axr_draw_params* params, axr_batch** out);
+/* Clear every view's framebuffer (bgra packed as in axr_fb_clear, depth)
+ * at the start of each launch, inside the same graph. */
+int axr_batch_set_clear(axr_batch* batch, int enable, uint32_t bgra, float depth);
 int axr_batch_launch(axr_batch* batch);    /* enqueue (graph replay); completes at axr_sync */
 int axr_batch_destroy(axr_batch* batch);
 

@@ -113,3 +113,43 @@ def assert_parity(rep, tag=""):
     ok = (rep["coverage_mismatch"] == 0 and rep["depth_mismatch"] == 0 and rep["id_mismatch"] == 0
           and rep["color_max"] <= 1)
     assert ok, f"parity failed {tag}: {rep}"
+
+
+class GpuBatch:
+    """A BatchWorkload made resident: one texture set, one mesh per workload
+    mesh, one framebuffer per view, and the batch handle."""
+
+    def __init__(self, ctx, wl, with_ids=True, tile=None):
+        if tile:
+            ctx.set_tile_size(tile)
+        self.ctx = ctx
+        self.wl = wl
+        self.textures = {k: R.Texture(ctx, t.rgba, t.filter) for k, t in wl.textures.items()}
+        m = wl.material
+        g = lambda n: self.textures[n] if n else None  # noqa: E731
+        mat = R.Material(g(m.diffuse), g(m.metallic), g(m.bump), g(m.roughness), g(m.ao), m.specular_exponent)
+        self.meshes = [R.Mesh(ctx, v, f, [R.MaterialGroup(0, 0, f.shape[0])], [mat]) for v, f in wl.meshes]
+        self.fbs = [R.Framebuffer(ctx, vs.width, vs.height, with_ids=with_ids) for vs in wl.views]
+        self.params = [params_of(wl.scene(i)) for i in range(len(wl.views))]
+        self.batch = R.Batch(ctx, self.fbs, [self.meshes[vs.mesh] for vs in wl.views], self.params)
+        self.batch.set_clear(BLACK)
+
+    def draw_solo(self, i, flags=0):
+        """View i through the plain per-frame path (clear + drawMesh)."""
+        import ctypes as C
+        fb = self.fbs[i]
+        fb.clear(BLACK)
+        p = self.params[i]
+        p.flags = flags
+        self.ctx.check(self.ctx._lib.axr_draw(self.ctx.handle, fb.handle, self.meshes[self.wl.views[i].mesh].handle,
+                                              C.byref(p)))
+        p.flags = 0
+
+    def close(self):
+        self.batch.close()
+        for fb in self.fbs:
+            fb.close()
+        for m in self.meshes:
+            m.close()
+        for t in self.textures.values():
+            t.close()
