@@ -55,7 +55,8 @@ class AxrDrawParams(C.Structure):
 
 
 class AxrConfig(C.Structure):
-    _fields_ = [("tile_size", C.c_int32), ("reserved", C.c_int32 * 7)]
+    _fields_ = [("tile_size", C.c_int32), ("raster_threads", C.c_int32), ("bin_capacity", C.c_int32),
+                ("small_footprint", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class AxrKernelStats(C.Structure):

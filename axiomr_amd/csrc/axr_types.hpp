@@ -118,7 +118,7 @@ enum Counter : uint32_t {
     CNT_CLIP_FACES = 0,   // faces queued for the full clipper
     CNT_CLIP_RECS = 1,    // clip records reserved
     CNT_OVF_PAIRS = 2,    // pairs placed in the overflow pool
-    CNT_UNUSED = 3,
+    CNT_ACTIVE = 3,       // non-empty bins listed by the scan
     CNT_ERROR = 4,        // sticky error bits, see ERR_*
     CNT_NEED_CLIP_FACES = 5,
     CNT_NEED_CLIP_RECS = 6,
@@ -139,19 +139,20 @@ struct Arena {
     uint32_t* tile_count;      // per bin
     uint32_t* tile_off;        // exclusive prefix of tile_count (+1 entry)
     uint4* pairs;              // (bin, slot in bin, record ref, 0) per overlap:
-                               // WAVE_PAIRS per setup wave, then the overflow pool
+                               // wave_cap per setup wave, then the overflow pool
     uint32_t* wave_pairs;      // pairs each setup wave wrote into its own region
     uint32_t* bins;            // refs, grouped by bin
-    uint32_t* scan_sums;       // per-scan-block partial sums
+    uint32_t* active;          // non-empty bins, in bin order
+    unsigned long long* scan_sums;   // per-scan-block partial sums (active << 32 | pairs)
     uint32_t cap_clip_faces;
     uint32_t cap_clip_recs;
     uint32_t cap_waves;        // setup waves with a region
+    uint32_t wave_cap;         // pairs per setup-wave region (a wave that needs
+                               // more takes them from the overflow pool with
+                               // a single atomic)
     uint32_t cap_ovf;          // overflow pool entries (after the regions)
     uint32_t cap_bins;
 };
-// Pair region of one setup wave: 2 per lane on average.  A wave that needs
-// more takes them from the overflow pool with a single atomic.
-constexpr uint32_t WAVE_PAIRS = 128;
 
 // Kernel arguments common to every stage.  `table` points to `nviews`
 // descriptors in device memory; when it is null the single descriptor `one`
@@ -160,10 +161,14 @@ struct KArgs {
     const ViewDesc* table;
     const uint32_t* face_bases;   // nviews + 1 prefix (device), batch only
     const uint32_t* tile_bases;   // nviews + 1 prefix (device), batch only
+    const uint32_t* tile_views;   // view of every bin (device), batch only
     uint32_t nviews;
     uint32_t total_faces;
     uint32_t total_tiles;
-    int32_t tile_size;
+    int32_t tile_size;          // bin edge in pixels: 8, 16 or 32
+    int32_t raster_threads;     // workgroup size of k_raster: 64, 128 or 256
+    int32_t small_footprint;    // a triangle covering at most this many pixels
+                                // of a bin is walked by one lane
     ViewDesc one;
 };
 

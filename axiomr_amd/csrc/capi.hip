@@ -41,7 +41,8 @@ struct axr_ctx {
     std::string err;
     axr_config cfg{};
     // frame arena
-    DevBuf counters, rec, clip_faces, clip_rec, clip_verts, tile_count, tile_off, pairs, wave_pairs, bins, scan_sums;
+    DevBuf counters, rec, clip_faces, clip_rec, clip_verts, tile_count, tile_off, pairs, wave_pairs, bins, active,
+        scan_sums;
     bool arena_dirty = true;   // bin counts / counters need a k_prep before the next frame
     Arena arena{};
     uint64_t arena_gen = 0;
@@ -90,6 +91,7 @@ struct axr_batch {
     uint32_t* d_face_bases = nullptr;
     uint32_t* d_tile_bases = nullptr;
     uint32_t* d_pix_bases = nullptr;
+    uint32_t* d_tile_views = nullptr;
     uint32_t total_pixels = 0;
     bool clear = false;            // clear every view's framebuffer before drawing
     uint32_t clear_bgra = 0;
@@ -190,10 +192,12 @@ int ensure_arena(axr_ctx* ctx, uint64_t faces, uint64_t tiles, uint64_t need_cli
     if ((r = grow(ctx, ctx->clip_verts, clip_recs * 3 * sizeof(Vertex56)))) return r;
     if ((r = grow(ctx, ctx->tile_count, (tiles + 1) * 4))) return r;
     if ((r = grow(ctx, ctx->tile_off, (tiles + 1) * 4))) return r;
-    if ((r = grow(ctx, ctx->pairs, (waves * WAVE_PAIRS + ovf) * 16))) return r;
+    const uint32_t wave_cap = (uint32_t)ctx->cfg.bin_capacity;
+    if ((r = grow(ctx, ctx->pairs, (waves * wave_cap + ovf) * 16))) return r;
     if ((r = grow(ctx, ctx->wave_pairs, waves * 4))) return r;
     if ((r = grow(ctx, ctx->bins, nbins * 4))) return r;
-    if ((r = grow(ctx, ctx->scan_sums, scan_blocks * 4))) return r;
+    if ((r = grow(ctx, ctx->active, (tiles + 1) * 4))) return r;
+    if ((r = grow(ctx, ctx->scan_sums, scan_blocks * 8))) return r;
     a.counters = (uint32_t*)ctx->counters.p;
     a.rec = (TriRec*)ctx->rec.p;
     a.clip_faces = (uint32_t*)ctx->clip_faces.p;
@@ -204,12 +208,16 @@ int ensure_arena(axr_ctx* ctx, uint64_t faces, uint64_t tiles, uint64_t need_cli
     a.pairs = (uint4*)ctx->pairs.p;
     a.wave_pairs = (uint32_t*)ctx->wave_pairs.p;
     a.bins = (uint32_t*)ctx->bins.p;
-    a.scan_sums = (uint32_t*)ctx->scan_sums.p;
+    a.active = (uint32_t*)ctx->active.p;
+    a.scan_sums = (unsigned long long*)ctx->scan_sums.p;
     a.cap_clip_faces = (uint32_t)std::min<uint64_t>(ctx->clip_faces.bytes / 4, 0xFFFFFFFFull);
     a.cap_clip_recs = (uint32_t)std::min<uint64_t>(
         {ctx->clip_rec.bytes / sizeof(TriRec), ctx->clip_verts.bytes / (3 * sizeof(Vertex56)), 0x7FFFFFFFull});
-    a.cap_waves = (uint32_t)std::min<uint64_t>(ctx->wave_pairs.bytes / 4, 0x7FFFFFFFull / WAVE_PAIRS);
-    a.cap_ovf = (uint32_t)std::min<uint64_t>(ctx->pairs.bytes / 16 - (uint64_t)a.cap_waves * WAVE_PAIRS, 0x7FFFFFFFull);
+    if (a.wave_cap != wave_cap) ++ctx->arena_gen;   // the pair layout changed: captured graphs are stale
+    a.wave_cap = wave_cap;
+    a.cap_waves = (uint32_t)std::min<uint64_t>(
+        {ctx->wave_pairs.bytes / 4, 0x7FFFFFFFull / wave_cap, ctx->pairs.bytes / 16 / wave_cap});
+    a.cap_ovf = (uint32_t)std::min<uint64_t>(ctx->pairs.bytes / 16 - (uint64_t)a.cap_waves * wave_cap, 0x7FFFFFFFull);
     a.cap_bins = (uint32_t)std::min<uint64_t>(ctx->bins.bytes / 4, 0x7FFFFFFFull);
     if (ctx->arena_gen != gen_start || ctx->tile_count.bytes / 4 < tiles + 1) ctx->arena_dirty = true;
     return AXR_OK;
@@ -362,6 +370,9 @@ int axr_ctx_create(int device, uint64_t hbm_budget_bytes, axr_ctx** out) {
     ctx->device = device;
     ctx->budget = hbm_budget_bytes;
     ctx->cfg.tile_size = 16;
+    ctx->cfg.raster_threads = 64;
+    ctx->cfg.bin_capacity = 128;
+    ctx->cfg.small_footprint = 12;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->host_counters, CNT_COUNT * 4, hipHostMallocDefault);
@@ -382,7 +393,8 @@ int axr_ctx_destroy(axr_ctx* ctx) {
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->counters, &ctx->rec, &ctx->clip_faces, &ctx->clip_rec, &ctx->clip_verts,
-                      &ctx->tile_count, &ctx->tile_off, &ctx->pairs, &ctx->wave_pairs, &ctx->bins, &ctx->scan_sums};
+                      &ctx->tile_count, &ctx->tile_off, &ctx->pairs, &ctx->wave_pairs, &ctx->bins, &ctx->active,
+                      &ctx->scan_sums};
     for (DevBuf* b : bufs) free_buf(ctx, *b);
     for (auto& ring : ctx->ev)
         for (auto& e : ring)
@@ -399,6 +411,12 @@ int axr_ctx_set_config(axr_ctx* ctx, const axr_config* cfg) {
     if (!ctx || !cfg) return AXR_ERR_INVALID;
     if (cfg->tile_size != 8 && cfg->tile_size != 16 && cfg->tile_size != 32)
         return fail(ctx, AXR_ERR_INVALID, "tile_size must be 8, 16 or 32 (got %d)", cfg->tile_size);
+    if (cfg->raster_threads != 64 && cfg->raster_threads != 128 && cfg->raster_threads != 256)
+        return fail(ctx, AXR_ERR_INVALID, "raster_threads must be 64, 128 or 256 (got %d)", cfg->raster_threads);
+    if (cfg->bin_capacity < 64 || cfg->bin_capacity > 1024)
+        return fail(ctx, AXR_ERR_INVALID, "bin_capacity must be in [64, 1024] (got %d)", cfg->bin_capacity);
+    if (cfg->small_footprint < 1 || cfg->small_footprint > 1024)
+        return fail(ctx, AXR_ERR_INVALID, "small_footprint must be in [1, 1024] (got %d)", cfg->small_footprint);
     ctx->cfg = *cfg;
     return AXR_OK;
 }
@@ -650,6 +668,8 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     ka.total_faces = mesh->nfaces_sub;
     ka.total_tiles = (uint32_t)(ka.one.tiles_x * ka.one.tiles_y);
     ka.tile_size = tile;
+    ka.raster_threads = ctx->cfg.raster_threads;
+    ka.small_footprint = ctx->cfg.small_footprint;
     int r = ensure_arena(ctx, ka.total_faces, ka.total_tiles);
     if (r) return r;
     const bool async = (p->flags & AXR_DRAW_ASYNC) != 0;
@@ -707,25 +727,33 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     tbase[nviews] = (uint32_t)tiles;
     pbase[nviews] = (uint32_t)pixels;
     b->total_pixels = (uint32_t)pixels;
+    std::vector<uint32_t> tview(tiles);
+    for (uint32_t i = 0; i < nviews; ++i)
+        for (uint32_t t = tbase[i]; t < tbase[i + 1]; ++t) tview[t] = i;
     int r;
     const size_t sz_v = nviews * sizeof(ViewDesc), sz_b = (nviews + 1) * 4;
     if ((r = dev_alloc(ctx, (void**)&b->d_views, sz_v)) || (r = dev_alloc(ctx, (void**)&b->d_face_bases, sz_b)) ||
         (r = dev_alloc(ctx, (void**)&b->d_tile_bases, sz_b)) || (r = dev_alloc(ctx, (void**)&b->d_pix_bases, sz_b)) ||
         (r = upload(ctx, b->d_views, b->views.data(), sz_v)) || (r = upload(ctx, b->d_face_bases, fbase.data(), sz_b)) ||
-        (r = upload(ctx, b->d_tile_bases, tbase.data(), sz_b)) || (r = upload(ctx, b->d_pix_bases, pbase.data(), sz_b))) {
+        (r = upload(ctx, b->d_tile_bases, tbase.data(), sz_b)) || (r = upload(ctx, b->d_pix_bases, pbase.data(), sz_b)) ||
+        (r = dev_alloc(ctx, (void**)&b->d_tile_views, tiles * 4)) ||
+        (r = upload(ctx, b->d_tile_views, tview.data(), tiles * 4))) {
         axr_batch_destroy(b);
         return r;
     }
-    b->bytes = sz_v + 3 * sz_b;
+    b->bytes = sz_v + 3 * sz_b + tiles * 4;
     ctx->in_use += b->bytes;
     KArgs& ka = b->ka;
     ka.table = b->d_views;
     ka.face_bases = b->d_face_bases;
     ka.tile_bases = b->d_tile_bases;
+    ka.tile_views = b->d_tile_views;
     ka.nviews = nviews;
     ka.total_faces = (uint32_t)faces;
     ka.total_tiles = (uint32_t)tiles;
     ka.tile_size = tile;
+    ka.raster_threads = ctx->cfg.raster_threads;
+    ka.small_footprint = ctx->cfg.small_footprint;
     ka.one = b->views[0];
     if ((r = ensure_arena(ctx, faces, tiles))) { axr_batch_destroy(b); return r; }
     *out = b;
@@ -791,6 +819,7 @@ int axr_batch_destroy(axr_batch* b) {
     if (b->d_face_bases) hipFree(b->d_face_bases);
     if (b->d_tile_bases) hipFree(b->d_tile_bases);
     if (b->d_pix_bases) hipFree(b->d_pix_bases);
+    if (b->d_tile_views) hipFree(b->d_tile_views);
     ctx->in_use -= b->bytes;
     delete b;
     return AXR_OK;

@@ -277,17 +277,17 @@ __device__ __forceinline__ void emit_pairs_wave(const Arena& ar, const ViewDesc&
     const uint32_t n = br.count();
     const uint32_t incl = wave_incl_scan(n);
     const uint32_t total = __shfl(incl, 63);
-    uint32_t base = wave * WAVE_PAIRS;
+    uint32_t base = wave * ar.wave_cap;
     bool fits = true;
-    if (total > WAVE_PAIRS) {
+    if (total > ar.wave_cap) {
         uint32_t o = 0;
         if (__lane_id() == 0) o = atomicAdd(&ar.counters[CNT_OVF_PAIRS], total);
         o = __shfl(o, 0);
         fits = o + total <= ar.cap_ovf;
         if (!fits && __lane_id() == 0) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, o + total);
-        base = ar.cap_waves * WAVE_PAIRS + o;
+        base = ar.cap_waves * ar.wave_cap + o;
     }
-    if (__lane_id() == 0) ar.wave_pairs[wave] = total > WAVE_PAIRS ? 0u : total;
+    if (__lane_id() == 0) ar.wave_pairs[wave] = total > ar.wave_cap ? 0u : total;
     if (total == 0) return;
     const uint32_t p0 = base + incl - n;
     for (uint32_t k = 0; __ballot(k < n); ++k) {
@@ -317,7 +317,7 @@ __device__ __forceinline__ void emit_pairs_lane(const Arena& ar, const ViewDesc&
     const uint32_t o = atomicAdd(&ar.counters[CNT_OVF_PAIRS], n);
     const bool fits = o + n <= ar.cap_ovf;
     if (!fits) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, o + n);
-    const uint32_t p0 = ar.cap_waves * WAVE_PAIRS + o;
+    const uint32_t p0 = ar.cap_waves * ar.wave_cap + o;
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t bin = br.bin(k, V);
         const uint32_t slot = atomicAdd(&ar.tile_count[bin], 1u);
@@ -517,99 +517,140 @@ __global__ __launch_bounds__(64) void k_clip(KArgs ka, Arena ar) {
 }
 
 // ---------------------------------------------------------------------------
-// prefix sum of the bin counts: block reduce, top-level scan, down-sweep.
+// prefix sum of the bin counts.  Each bin contributes the 64-bit value
+// (count > 0) << 32 | count, so one scan gives both the bin's offset in the
+// bins array (low word) and its position in the list of non-empty bins
+// (high word), which the raster walks instead of every bin.
+
+typedef unsigned long long u64;
+
+__device__ __forceinline__ u64 shfl_up64(u64 v, int o) {
+    uint32_t lo = __shfl_up((uint32_t)v, o), hi = __shfl_up((uint32_t)(v >> 32), o);
+    return ((u64)hi << 32) | lo;
+}
+
+__device__ __forceinline__ u64 wave_incl_scan64(u64 v) {
+    const int lane = (int)__lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        u64 t = shfl_up64(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
 
 // Exclusive scan across the 1024 threads of a block; returns the prefix and
 // the block total.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+__device__ __forceinline__ u64 block_excl_scan64(u64 v, u64* lds, u64& total) {
     const int lane = (int)__lane_id(), wave = threadIdx.x >> 6;
-    uint32_t incl = wave_incl_scan(v);
+    u64 incl = wave_incl_scan64(v);
     if (lane == 63) lds[wave] = incl;
     __syncthreads();
     if (wave == 0) {
-        uint32_t w = lane < (SCAN_THREADS / 64) ? lds[lane] : 0u;
-        uint32_t wi = wave_incl_scan(w);
+        u64 w = lane < (SCAN_THREADS / 64) ? lds[lane] : 0ull;
+        u64 wi = wave_incl_scan64(w);
         if (lane < (SCAN_THREADS / 64)) lds[16 + lane] = wi - w;
         if (lane == (SCAN_THREADS / 64) - 1) lds[32] = wi;
     }
     __syncthreads();
     total = lds[32];
-    uint32_t r = lds[16 + wave] + incl - v;
+    u64 r = lds[16 + wave] + incl - v;
     __syncthreads();
     return r;
 }
 
+__device__ __forceinline__ u64 bin_value(uint32_t count) { return ((u64)(count != 0) << 32) | count; }
+
+// Write the offsets of SCAN_ITEMS consecutive bins starting at `base` and
+// list the non-empty ones.
+__device__ __forceinline__ void write_offsets(const Arena& ar, uint32_t base, uint32_t ntiles, const uint32_t* c, u64 ex) {
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < ntiles) {
+            ar.tile_off[base + k] = (uint32_t)ex;
+            if (c[k]) ar.active[ex >> 32] = base + k;
+        }
+        ex += bin_value(c[k]);
+    }
+}
+
+// The per-frame counters the scan is the last to need: the clip queue and
+// the clip records have been consumed by now.
+__device__ __forceinline__ void reset_consumed_counters(const Arena& ar) {
+    ar.counters[CNT_CLIP_FACES] = 0;
+    ar.counters[CNT_CLIP_RECS] = 0;
+}
+
+__device__ __forceinline__ void finish_scan(const Arena& ar, u64 total) {
+    const uint32_t pairs = (uint32_t)total;
+    ar.counters[CNT_ACTIVE] = (uint32_t)(total >> 32);
+    reset_consumed_counters(ar);
+    if (pairs > ar.cap_bins) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, pairs);
+}
+
 // One block scans all bin counts (used when there are at most
-// SMALL_SCAN_TILES bins); also checks the bins capacity.
+// SMALL_SCAN_TILES bins).
 constexpr uint32_t SMALL_SCAN_TILES = 65536;
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_single(Arena ar, uint32_t ntiles) {
-    __shared__ uint32_t lds[64];
-    __shared__ uint32_t carry;
+    __shared__ u64 lds[64];
+    __shared__ u64 carry;
     if (ar.counters[CNT_ERROR]) return;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (uint32_t start = 0; start < ntiles; start += SCAN_BLOCK) {
         const uint32_t base = start + threadIdx.x * SCAN_ITEMS;
-        uint32_t v[SCAN_ITEMS];
-        uint32_t s = 0;
-        for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (base + k < ntiles) ? ar.tile_count[base + k] : 0u; s += v[k]; }
-        uint32_t total;
-        uint32_t ex = block_excl_scan(s, lds, total) + carry;
-        for (int k = 0; k < SCAN_ITEMS; ++k) {
-            if (base + k < ntiles) ar.tile_off[base + k] = ex;
-            ex += v[k];
-        }
+        uint32_t c[SCAN_ITEMS];
+        u64 s = 0;
+        for (int k = 0; k < SCAN_ITEMS; ++k) { c[k] = (base + k < ntiles) ? ar.tile_count[base + k] : 0u; s += bin_value(c[k]); }
+        u64 total;
+        const u64 ex = block_excl_scan64(s, lds, total) + carry;
+        write_offsets(ar, base, ntiles, c, ex);
         __syncthreads();
         if (threadIdx.x == 0) carry += total;
         __syncthreads();
     }
-    if (threadIdx.x == 0 && carry > ar.cap_bins) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, carry);
+    if (threadIdx.x == 0) finish_scan(ar, carry);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Arena ar, uint32_t ntiles) {
-    __shared__ uint32_t lds[64];
+    __shared__ u64 lds[64];
     if (ar.counters[CNT_ERROR]) return;
     const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
-    uint32_t s = 0;
-    for (int k = 0; k < SCAN_ITEMS; ++k) if (base + k < ntiles) s += ar.tile_count[base + k];
-    uint32_t total;
-    block_excl_scan(s, lds, total);
+    u64 s = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) if (base + k < ntiles) s += bin_value(ar.tile_count[base + k]);
+    u64 total;
+    block_excl_scan64(s, lds, total);
     if (threadIdx.x == 0) ar.scan_sums[blockIdx.x] = total;
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(Arena ar, uint32_t nblocks) {
-    __shared__ uint32_t lds[64];
-    __shared__ uint32_t carry;
+    __shared__ u64 lds[64];
+    __shared__ u64 carry;
     if (ar.counters[CNT_ERROR]) return;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (uint32_t start = 0; start < nblocks; start += SCAN_THREADS) {
         const uint32_t i = start + threadIdx.x;
-        uint32_t v = i < nblocks ? ar.scan_sums[i] : 0u;
-        uint32_t total;
-        uint32_t ex = block_excl_scan(v, lds, total);
-        const uint32_t c = carry;
+        u64 v = i < nblocks ? ar.scan_sums[i] : 0ull;
+        u64 total;
+        u64 ex = block_excl_scan64(v, lds, total);
+        const u64 c = carry;
         if (i < nblocks) ar.scan_sums[i] = c + ex;
         __syncthreads();
         if (threadIdx.x == 0) carry = c + total;
         __syncthreads();
     }
-    if (threadIdx.x == 0 && carry > ar.cap_bins) set_error(ar, ERR_PAIRS, CNT_NEED_PAIRS, carry);
+    if (threadIdx.x == 0) finish_scan(ar, carry);
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Arena ar, uint32_t ntiles) {
-    __shared__ uint32_t lds[64];
+    __shared__ u64 lds[64];
     if (ar.counters[CNT_ERROR]) return;
     const uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
-    for (int k = 0; k < SCAN_ITEMS; ++k) { v[k] = (base + k < ntiles) ? ar.tile_count[base + k] : 0u; s += v[k]; }
-    uint32_t total;
-    uint32_t ex = block_excl_scan(s, lds, total) + ar.scan_sums[blockIdx.x];
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        if (base + k < ntiles) ar.tile_off[base + k] = ex;
-        ex += v[k];
-    }
+    uint32_t c[SCAN_ITEMS];
+    u64 s = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) { c[k] = (base + k < ntiles) ? ar.tile_count[base + k] : 0u; s += bin_value(c[k]); }
+    u64 total;
+    const u64 ex = block_excl_scan64(s, lds, total) + ar.scan_sums[blockIdx.x];
+    write_offsets(ar, base, ntiles, c, ex);
 }
 
 // ---------------------------------------------------------------------------
@@ -619,15 +660,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Arena ar, uint32_t n
 __global__ __launch_bounds__(256) void k_scatter(Arena ar, uint32_t nwaves) {
     if (ar.counters[CNT_ERROR]) return;
     const uint32_t stride = gridDim.x * 256u;
-    const uint32_t nreg = nwaves * WAVE_PAIRS;
+    const uint32_t cap = ar.wave_cap;
+    const uint32_t nreg = nwaves * cap;
     for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < nreg; q += stride) {
-        if ((q % WAVE_PAIRS) < ar.wave_pairs[q / WAVE_PAIRS]) {
+        if ((q % cap) < ar.wave_pairs[q / cap]) {
             const uint4 p = ar.pairs[q];
             ar.bins[ar.tile_off[p.x] + p.y] = p.z;
         }
     }
     const uint32_t novf = imin((int)ar.counters[CNT_OVF_PAIRS], (int)ar.cap_ovf);
-    const uint4* ovf = ar.pairs + (size_t)ar.cap_waves * WAVE_PAIRS;
+    const uint4* ovf = ar.pairs + (size_t)ar.cap_waves * ar.wave_cap;
     for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < novf; q += stride) {
         const uint4 p = ovf[q];
         ar.bins[ar.tile_off[p.x] + p.y] = p.z;
@@ -637,9 +679,6 @@ __global__ __launch_bounds__(256) void k_scatter(Arena ar, uint32_t nwaves) {
 // ---------------------------------------------------------------------------
 // k_raster: one workgroup per bin.
 
-// Pixels at or below this count are walked by one lane; bigger footprints
-// are spread over the whole workgroup.
-constexpr int SMALL_FOOTPRINT = 12;
 
 __device__ __forceinline__ const TriRec& rec_of(const Arena& ar, uint32_t ref) {
     return (ref & REF_CLIP) ? ar.clip_rec[ref & ~REF_CLIP] : ar.rec[ref];
@@ -704,28 +743,21 @@ __device__ __forceinline__ void resolve_tile(const ViewDesc& V, const Arena& ar,
 }
 
 template <int T, int NT, bool BATCH>
-__global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
-    __shared__ unsigned long long keys[T * T];
-    __shared__ uint32_t bigq[NT];
-    __shared__ uint32_t nbig;
-    if (ar.counters[CNT_ERROR]) return;
-    const uint32_t t = blockIdx.x;
+__device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uint32_t t, unsigned long long* keys,
+                                           uint32_t* bigq, uint32_t& nbig) {
     uint32_t vi = 0;
-    if constexpr (BATCH) vi = find_le(ka.tile_bases, ka.nviews, t);
+    if constexpr (BATCH) vi = ka.tile_views[t];
     const ViewDesc& V = view_at<BATCH>(ka, vi);
-    if (t == 0 && threadIdx.x < CNT_ERROR) ar.counters[threadIdx.x] = 0;   // per-frame counters, consumed
     const uint32_t n = ar.tile_count[t];
-    if (n == 0) return;
     const uint32_t off = ar.tile_off[t];
-    __syncthreads();
-    if (threadIdx.x == 0) ar.tile_count[t] = 0;   // consumed: leave the arena clean for the next frame
     const uint32_t local = t - V.tile_base;
     const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
     const int ex = imin(ox + T, V.W), ey = imin(oy + T, V.H);
-
     for (int p = threadIdx.x; p < T * T; p += NT) keys[p] = EMPTY_KEY;
     if (threadIdx.x == 0) nbig = 0;
     __syncthreads();
+    // every wave has read n: the count is consumed, leave it zero for the next frame
+    if (threadIdx.x == 0) ar.tile_count[t] = 0;
 
     for (uint32_t b0 = 0; b0 < n; b0 += NT) {
         const uint32_t i = b0 + threadIdx.x;
@@ -738,7 +770,7 @@ __global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
                 const int y0 = imax(oy, rt.fy0), y1 = imin(ey, rt.cy1);
                 const int w = x1 - x0, h = y1 - y0;
                 if (w > 0 && h > 0) {
-                    if (w * h <= SMALL_FOOTPRINT) {
+                    if (w * h <= ka.small_footprint) {
                         const uint32_t key = r.c.y;
                         for (int py = y0; py < y1; ++py)
                             for (int px = x0; px < x1; ++px) {
@@ -782,6 +814,29 @@ __global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
     case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy); break;
     default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy); break;
     }
+}
+
+// One workgroup per entry of the list of non-empty bins (the grid covers
+// every bin; blocks past the list exit after one load).  Workgroups are
+// dispatched to the 8 XCDs round-robin, so block ids are permuted to give
+// each XCD a contiguous run of the list: neighbouring bins share triangles,
+// vertices and texels in that XCD's L2.
+constexpr uint32_t NUM_XCD = 8;
+
+template <int T, int NT, bool BATCH>
+__global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
+    __shared__ unsigned long long keys[T * T];
+    __shared__ uint32_t bigq[NT];
+    __shared__ uint32_t nbig;
+    if (ar.counters[CNT_ERROR]) return;
+    // the pair overflow count was last needed by the scatter
+    if (blockIdx.x == 0 && threadIdx.x == 0) ar.counters[CNT_OVF_PAIRS] = 0;
+    const uint32_t nactive = ar.counters[CNT_ACTIVE];
+    const uint32_t chunk = (nactive + NUM_XCD - 1) / NUM_XCD;
+    const uint32_t j = blockIdx.x / NUM_XCD;
+    const uint32_t a = (blockIdx.x % NUM_XCD) * chunk + j;
+    if (j >= chunk || a >= nactive) return;
+    raster_bin<T, NT, BATCH>(ka, ar, ar.active[a], keys, bigq, nbig);
 }
 
 // ---------------------------------------------------------------------------
@@ -837,21 +892,35 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
     }
     if (ev) (void)hipEventRecord(ev[2], st);
     const uint32_t nwaves = setup_blocks * 4;
-    const uint32_t sblocks = imax(64, imin(2048, (int)((nwaves * WAVE_PAIRS / 8 + 255) / 256)));
+    const uint32_t sblocks = imax(64, imin(2048, (int)((nwaves * ar.wave_cap / 8 + 255) / 256)));
     k_scatter<<<sblocks, 256, 0, st>>>(ar, nwaves);
     if (ev) (void)hipEventRecord(ev[3], st);
-    k_raster<T, NT, BATCH><<<imax(1, (int)ntiles), NT, 0, st>>>(ka, ar);
+    const uint32_t rblocks = ((imax(1, (int)ntiles) + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
+    k_raster<T, NT, BATCH><<<rblocks, NT, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[4], st);
     return hipGetLastError();
 }
 
-hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
-    const bool batch = ka.table != nullptr;
-    switch (ka.tile_size) {
-    case 8: return batch ? launch_frame_t<8, 64, true>(ka, ar, st, ev) : launch_frame_t<8, 64, false>(ka, ar, st, ev);
-    case 32: return batch ? launch_frame_t<32, 256, true>(ka, ar, st, ev) : launch_frame_t<32, 256, false>(ka, ar, st, ev);
-    default: return batch ? launch_frame_t<16, 256, true>(ka, ar, st, ev) : launch_frame_t<16, 256, false>(ka, ar, st, ev);
+template <int T, bool BATCH>
+static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+    switch (ka.raster_threads) {
+    case 64: return launch_frame_t<T, 64, BATCH>(ka, ar, st, ev);
+    case 128: return launch_frame_t<T, 128, BATCH>(ka, ar, st, ev);
+    default: return launch_frame_t<T, 256, BATCH>(ka, ar, st, ev);
     }
+}
+
+template <bool BATCH>
+static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+    switch (ka.tile_size) {
+    case 8: return launch_frame_nt<8, BATCH>(ka, ar, st, ev);
+    case 32: return launch_frame_nt<32, BATCH>(ka, ar, st, ev);
+    default: return launch_frame_nt<16, BATCH>(ka, ar, st, ev);
+    }
+}
+
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+    return ka.table ? launch_frame_b<true>(ka, ar, st, ev) : launch_frame_b<false>(ka, ar, st, ev);
 }
 
 hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,

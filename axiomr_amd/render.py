@@ -70,15 +70,27 @@ class Context:
     def sync(self) -> None:
         self.check(self._lib.axr_sync(self.handle))
 
-    def set_tile_size(self, tile: int) -> None:
-        cfg = N.AxrConfig()
-        cfg.tile_size = tile
-        self.check(self._lib.axr_ctx_set_config(self.handle, C.byref(cfg)))
-
-    def tile_size(self) -> int:
+    def config(self) -> dict:
+        """Tunables of the binning pipeline (see axr_config in include/axiomr.h)."""
         cfg = N.AxrConfig()
         self.check(self._lib.axr_ctx_get_config(self.handle, C.byref(cfg)))
-        return cfg.tile_size
+        return {"tile_size": cfg.tile_size, "raster_threads": cfg.raster_threads, "bin_capacity": cfg.bin_capacity,
+                "small_footprint": cfg.small_footprint}
+
+    def set_config(self, **kw) -> None:
+        cfg = N.AxrConfig()
+        self.check(self._lib.axr_ctx_get_config(self.handle, C.byref(cfg)))
+        for k, v in kw.items():
+            if not hasattr(cfg, k) or k == "reserved":
+                raise KeyError(k)
+            setattr(cfg, k, int(v))
+        self.check(self._lib.axr_ctx_set_config(self.handle, C.byref(cfg)))
+
+    def set_tile_size(self, tile: int) -> None:
+        self.set_config(tile_size=tile)
+
+    def tile_size(self) -> int:
+        return self.config()["tile_size"]
 
     def set_profiling(self, on: bool) -> None:
         self.check(self._lib.axr_ctx_set_profiling(self.handle, 1 if on else 0))

@@ -53,13 +53,25 @@ def setup_bytes(nverts, nfaces):
     return 12 * nfaces + 12 * nverts
 
 
-def run_plain(scene, steps, warmup, tile):
-    import harness as H
+def apply_config(ctx, workload, tile, use_tuned):
+    """The tuned tunables for `workload` when cached (and asked for), else
+    the defaults with --tile."""
+    from axiomr_amd import autotune as AT
+    cached = AT.load(workload) if use_tuned else None
+    if cached:
+        ctx.set_config(**cached["config"])
+    elif tile:
+        ctx.set_config(tile_size=tile)
+    return ctx.config()
+
+
+def run_plain(scene, steps, warmup, tile, use_tuned):
     import axiomr_amd as axr
     from axiomr_amd import render as R
+    from axiomr_amd import resident as H
 
     ctx = axr.Context(0)
-    ctx.set_tile_size(tile)
+    cfg = apply_config(ctx, "plain", tile, use_tuned)
     gs = H.GpuScene(ctx, scene)
     fb = R.Framebuffer(ctx, scene.width, scene.height)
     params = H.params_of(scene, flags=1)   # AXR_DRAW_ASYNC
@@ -103,28 +115,39 @@ def run_plain(scene, steps, warmup, tile):
     gs.close()
     ctx.close()
     return dict(seconds=dt, covered=covered, stages_ms=stages, frame_event_ms=frame_ms, mem_in_use=in_use,
-                arena_bytes=arena)
+                arena_bytes=arena, config=cfg)
 
 
-def run_batch(wl, reps, tile):
+def run_batch(wl, reps, tile, use_tuned):
     """The batched form (256 views, 32 meshes) against the plain per-frame
-    path on the same views: frames (views) per second of each."""
-    import harness as H
+    path on the same views: frames (views) per second of each.  Both use the
+    same tunables (the batch's tuned ones when cached)."""
     import axiomr_amd as axr
+    from axiomr_amd import resident as H
 
     ctx = axr.Context(0)
-    gb = H.GpuBatch(ctx, wl, with_ids=False, tile=tile)
-    n = len(wl.views)
-    # plain per-frame path: clear + drawMesh for every view, queued
-    for i in range(n):
-        gb.draw_solo(i, flags=1)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
+    # plain per-frame path: clear + drawMesh for every view, queued; timed
+    # under both tuned configurations, the faster one is the baseline
+    plain_s, plain_cfg = None, None
+    gb = None
+    for which in ("plain", "batch"):
+        cfg = apply_config(ctx, which, tile, use_tuned)
+        if gb is None:
+            gb = H.GpuBatch(ctx, wl, with_ids=False)
+        n = len(wl.views)
         for i in range(n):
             gb.draw_solo(i, flags=1)
-    ctx.sync()
-    plain_s = (time.perf_counter() - t0) / reps
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for i in range(n):
+                gb.draw_solo(i, flags=1)
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / reps
+        if plain_s is None or dt < plain_s:
+            plain_s, plain_cfg = dt, cfg
+    cfg = apply_config(ctx, "batch", tile, use_tuned)
+    gb.rebuild_batch()
     # batched: one graph launch draws (and clears) every view
     gb.batch.launch()
     ctx.sync()
@@ -143,14 +166,14 @@ def run_batch(wl, reps, tile):
     ctx.close()
     return dict(views=n, plain_views_per_s=n / plain_s, batch_views_per_s=n / batch_s, speedup=plain_s / batch_s,
                 batch_ms=batch_s * 1e3, plain_ms=plain_s * 1e3, stages_ms=stages,
-                faces=wl.total_faces, pixels=wl.total_pixels)
+                faces=wl.total_faces, pixels=wl.total_pixels, config=cfg, plain_config=plain_cfg)
 
 
 def cpu_baseline(scene, budget_s=12.0, threads=16):
     """The CPU checker (restated reference tiled pipeline with its thread
     pool) on the same frame, repeated until ~budget_s of CPU work."""
-    import harness as H
     import oracle as O
+    from axiomr_amd import resident as H
     threads = min(threads, os.cpu_count() or 1)
     osc = O.OracleScene(scene)
     rr = O.OracleRenderer(threads)
@@ -179,7 +202,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
-    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--tile", type=int, default=0, help="bin edge when no tuned config is used")
+    ap.add_argument("--untuned", action="store_true", help="ignore axiomr_amd/tuning/mi355x.json")
     ap.add_argument("--n", type=int, default=224, help="icosphere frequency (20 n^2 faces)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -198,7 +222,7 @@ def main():
     log(f"[bench] scene {scene.name}: {scene.nfaces} faces, {scene.vertices.shape[0]} vertices, "
         f"checksum {scene.checksum()} ({time.perf_counter() - t:.1f}s)")
 
-    r = run_plain(scene, args.steps, args.warmup, args.tile)
+    r = run_plain(scene, args.steps, args.warmup, args.tile, not args.untuned)
     fps = args.steps / r["seconds"]
     ms = 1000.0 * r["seconds"] / args.steps
     nverts, nfaces = scene.vertices.shape[0], scene.nfaces
@@ -223,7 +247,7 @@ def main():
         wl = S.batch_workload()
         log(f"[bench] batch workload: {len(wl.views)} views, {wl.total_faces} faces, {wl.total_pixels} pixels "
             f"({time.perf_counter() - t:.1f}s)")
-        b = run_batch(wl, args.batch_reps, args.tile)
+        b = run_batch(wl, args.batch_reps, args.tile, not args.untuned)
         log(f"[bench] batch: plain {b['plain_views_per_s']:.0f} views/s, batched {b['batch_views_per_s']:.0f} views/s, "
             f"x{b['speedup']:.2f}; stages {b['stages_ms']}")
         batch = {"workload": f"{len(wl.views)} views of 32 displaced icospheres (5k-500k triangles, log-uniform), "
@@ -233,7 +257,8 @@ def main():
                  "plain_views_per_s": round(b["plain_views_per_s"], 1),
                  "speedup_vs_plain": round(b["speedup"], 3),
                  "batched_gtri_per_s": round(b["faces"] / (b["batch_ms"] * 1e-3) / 1e9, 3),
-                 "batch_ms": round(b["batch_ms"], 3), "stages_ms": {k: round(v, 4) for k, v in b["stages_ms"].items()}}
+                 "batch_ms": round(b["batch_ms"], 3), "stages_ms": {k: round(v, 4) for k, v in b["stages_ms"].items()},
+                 "tunables": b["config"], "plain_tunables": b["plain_config"]}
 
     cpu = None
     if not args.no_cpu:
@@ -259,7 +284,7 @@ def main():
         "config": {"workload": f"plain per-frame GPU path, 1920x1080, displaced icosphere n={args.n} "
                                f"({nfaces} triangles), Phong + 2048^2 albedo + 2048^2 normal map (bilinear), "
                                f"3 point lights, FP32 depth, BGRA8",
-                   "tile": args.tile, "covered_pixels": r["covered"]},
+                   "tunables": r["config"], "covered_pixels": r["covered"]},
         "gtri_per_s": round(fps * nfaces / 1e9, 4),
         "frame_algorithmic_bytes": frame_bytes,
         "frame_roofline_frac": round(frame_bytes * fps / (HBM_PEAK_GBS * 1e9), 5),

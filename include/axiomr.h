@@ -101,10 +101,17 @@ typedef struct axr_draw_params {
     uint32_t flags;              /* AXR_DRAW_ASYNC */
 } axr_draw_params;
 
-/* Tunables of the binning pipeline (the autotuner writes these). */
+/* Tunables of the binning pipeline (the autotuner writes these).  None of
+ * them changes a pixel: coverage always follows the reference's 16-pixel
+ * tiles. */
 typedef struct axr_config {
-    int32_t tile_size;           /* GPU bin size: 8, 16 or 32 pixels */
-    int32_t reserved[7];
+    int32_t tile_size;           /* GPU bin edge: 8, 16 or 32 pixels (default 16) */
+    int32_t raster_threads;      /* threads per bin workgroup: 64, 128 or 256 (default 64) */
+    int32_t bin_capacity;        /* (bin, triangle) pairs each setup wave keeps in its own
+                                    region, 64..1024 (default 128); more go to a shared pool */
+    int32_t small_footprint;     /* triangles covering at most this many pixels of a bin are
+                                    walked by one lane, 1..1024 (default 12) */
+    int32_t reserved[4];
 } axr_config;
 
 /* Stage timing collected with HIP events (recorded on the context's stream,
