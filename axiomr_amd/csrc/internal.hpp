@@ -1,0 +1,121 @@
+// internal.hpp - host-side objects behind the C ABI handles, shared by the
+// context/draw/batch code (capi.hip) and the render server (serve.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/axiomr.h"
+#include "axr_types.hpp"
+
+namespace axr {
+
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev);
+hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st);
+hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st);
+hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
+                              uint32_t bgra, float d, hipStream_t st);
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// Where a failing call leaves its message.  The context has one (read by
+// axr_last_error); the server's scheduler thread has its own.
+struct ErrSink {
+    std::string msg;
+};
+
+// The per-frame working memory of the binning pipeline plus the stream its
+// kernels run on.  A context owns one; each lane of a render server owns one,
+// so batches on different lanes overlap on the device.
+struct FrameArena {
+    hipStream_t stream = nullptr;
+    DevBuf counters, rec, clip_faces, clip_rec, clip_verts, tile_count, tile_off, pairs, wave_pairs, bins, active,
+        scan_sums;
+    Arena a{};
+    uint64_t gen = 0;          // bumped whenever a buffer moves: captured graphs are stale
+    bool dirty = true;         // bin counts / counters need a k_prep before the next frame
+    uint32_t* host_counters = nullptr;   // pinned, CNT_COUNT words
+};
+
+}  // namespace axr
+
+struct axr_ctx {
+    int device = 0;
+    uint64_t budget = 0;          // 0 = unlimited
+    uint64_t in_use = 0;          // resource bytes (meshes, textures, framebuffers, batches, server pools)
+    uint64_t arena_bytes = 0;     // all frame arenas of this context
+    std::mutex mu;                // guards the two byte counters
+    axr::ErrSink err;
+    axr_config cfg{};
+    axr::FrameArena main;
+    bool profiling = false;
+    static constexpr int kEventRing = 64;
+    hipEvent_t ev[kEventRing][6] = {};
+    int ev_head = 0, ev_pending = 0;
+    axr_kernel_stats stats{};
+};
+
+struct axr_texture {
+    axr_ctx* ctx;
+    axr::DevTexture dev;
+    size_t bytes;
+};
+
+struct axr_mesh {
+    axr_ctx* ctx;
+    float* pos = nullptr;
+    axr::Vertex56* verts = nullptr;
+    uint32_t* idx = nullptr;
+    uint32_t* face_colors = nullptr;
+    axr::DevGroup* groups = nullptr;
+    axr::DevMaterial* materials = nullptr;
+    uint32_t ngroups = 0, nmaterials = 0;
+    uint64_t nverts = 0, nfaces = 0;
+    uint32_t nfaces_sub = 0;   // faces in submission order
+    size_t bytes = 0;
+};
+
+struct axr_fb {
+    axr_ctx* ctx;
+    int w, h;
+    uint32_t* color = nullptr;
+    float* depth = nullptr;
+    uint32_t* ids = nullptr;
+    size_t bytes = 0;
+};
+
+namespace axr {
+
+int fail(ErrSink& err, int code, const char* fmt, ...);
+int hip_fail(ErrSink& err, hipError_t e, const char* what);
+// Device allocation charged to the context's HBM budget.
+int dev_alloc(axr_ctx* ctx, ErrSink& err, void** p, size_t bytes);
+void dev_free(axr_ctx* ctx, void* p, size_t bytes);
+int upload(ErrSink& err, hipStream_t st, void* dst, const void* src, size_t bytes);
+
+int arena_create(axr_ctx* ctx, ErrSink& err, FrameArena& fa);
+void arena_destroy(axr_ctx* ctx, FrameArena& fa);
+// Size the arena for `faces` face slots and `tiles` bins; `need_*` carry
+// high-water marks the kernels reported after an overflow.
+int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uint64_t tiles, uint32_t wave_cap,
+                 uint64_t need_clip_faces = 0, uint64_t need_clip_recs = 0, uint64_t need_pairs = 0);
+int arena_read_counters(ErrSink& err, FrameArena& fa);
+// k_prep if the arena is fresh or dirty, then the frame's kernels; ev (may
+// be null) receives the 5 stage events.
+int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev);
+// After arena_read_counters: AXR_OK, or grow the arena to the reported
+// high-water marks, reset it and return AXR_ERR_OVERFLOW.
+int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uint64_t tiles,
+                         uint32_t wave_cap);
+
+// The uniforms drawMesh sets (reference tiled_pipeline.cpp:148-155).
+ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,
+                   const axr_draw_params* p, int tile);
+bool shader_ok(const axr_draw_params* p);
+
+}  // namespace axr
