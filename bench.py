@@ -29,6 +29,10 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 METRIC = "frames/s + Gtri/s @1080p 1M-tri vs plain GPU path; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch measured with rocprofv3 --pmc (FETCH_SIZE, then
+# WRITE_SIZE) on this workload, scaled by a plain-copy calibration; written
+# by `python -m axiomr_amd.rocprof_summary traffic`.
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_icosphere1080p.json"
 
 
 def log(msg):
@@ -197,6 +201,34 @@ def cpu_baseline(scene, budget_s=12.0, threads=16):
                 phases_s={k: v / len(times) for k, v in phases.items()})
 
 
+def measured_traffic(kname, tunables):
+    """HBM bytes per launch of `kname` from the committed --pmc summary, if
+    it was taken with the same tunables (else None)."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    data = json.loads(TRAFFIC_FILE.read_text())
+    if data.get("tunables") != tunables:
+        return None
+    hits = [v for k, v in data.get("kernels", {}).items() if k.startswith(kname + "<") or k == kname]
+    if len(hits) != 1:
+        return None
+    return int(round(hits[0]["traffic_bytes"]))
+
+
+def copy_calibration(mib, reps=5):
+    """A streaming read+write kernel over `mib` MiB (int32 add, torch's
+    elementwise kernel): under rocprofv3 --pmc its FETCH_SIZE and WRITE_SIZE
+    per dispatch each correspond to exactly mib MiB."""
+    import torch
+    n = (mib << 20) // 4
+    a = torch.ones(n, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.add(a, 1, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,12 +241,17 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain-copy kernel used to scale the --pmc byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described but not run", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from axiomr_amd import scenes as S
     t = time.perf_counter()
@@ -290,7 +327,8 @@ def main():
         "frame_roofline_frac": round(frame_bytes * fps / (HBM_PEAK_GBS * 1e9), 5),
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic(kname, r["config"]),
                      "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 5)},
         "cpu_baseline": cpu,
         "batched": batch,
