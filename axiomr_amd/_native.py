@@ -65,6 +65,29 @@ class AxrKernelStats(C.Structure):
     STAGES = ("setup", "clip", "scan", "scatter", "raster")
 
 
+class AxrServerConfig(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("max_batch_views", C.c_uint32), ("max_batch_faces", C.c_uint64),
+                ("max_batch_tiles", C.c_uint64), ("hbm_budget", C.c_uint64), ("clear_bgra", C.c_uint32),
+                ("with_ids", C.c_int32)]
+
+
+class AxrServeRequest(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("params", AxrDrawParams)]
+
+
+class AxrServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "failed", "rejected", "batches",
+                                           "batched_views", "max_views_in_batch", "launches_overlapped")] + [
+        ("max_lanes_busy", C.c_uint32), ("reserved0", C.c_uint32)] + [
+        (n, C.c_uint64) for n in ("deferred", "fb_reserved_bytes", "fb_peak_bytes", "fb_in_use_bytes",
+                                  "fixed_bytes", "queue_peak", "overflow_retries")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved0"}
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 1, 2, 3, 4
+
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 _F = C.POINTER(C.c_float)
@@ -98,6 +121,17 @@ SIGNATURES = {
     "axr_batch_set_clear": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_float]),
     "axr_batch_launch": (C.c_int, [_P]),
     "axr_batch_destroy": (C.c_int, [_P]),
+    "axr_server_create": (C.c_int, [_P, C.POINTER(AxrServerConfig), _PP]),
+    "axr_server_destroy": (C.c_int, [_P]),
+    "axr_server_last_error": (C.c_char_p, [_P]),
+    "axr_server_submit": (C.c_int, [_P, C.POINTER(AxrServeRequest), C.POINTER(C.c_uint64)]),
+    "axr_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int)]),
+    "axr_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double]),
+    "axr_server_read": (C.c_int, [_P, C.c_uint64, _U8, _F, _U32]),
+    "axr_server_release": (C.c_int, [_P, C.c_uint64]),
+    "axr_server_get_stats": (C.c_int, [_P, C.POINTER(AxrServerStats)]),
+    "axr_server_replay": (C.c_int, [_P, C.c_uint32, C.POINTER(AxrServeRequest), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
     "axr_gen_uv_sphere": (C.c_int, [C.c_int, C.c_int, C.c_float, _F, _U32]),
     "axr_gen_icosphere": (C.c_int, [C.c_int, C.c_float, C.c_uint64, _F, _U32]),
     "axr_gen_soup": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,

@@ -42,12 +42,12 @@ struct Block {
     int cls = -1;
 };
 
-// Size classes: 64 KiB * 2^(c/4), rounded up to 64 KiB; at most 19% of a
-// block is slack.
+// Size classes: 64 KiB * 2^(c/4), rounded up to 4 KiB (strictly
+// increasing); at most 19% of a block is slack.
 constexpr int kClasses = 96;
 inline uint64_t class_bytes(int c) {
     const double b = 65536.0 * std::pow(2.0, c / 4.0);
-    return ((uint64_t)std::ceil(b / 65536.0)) * 65536ull;
+    return ((uint64_t)std::ceil(b / 4096.0)) * 4096ull;
 }
 inline int class_of(uint64_t bytes) {
     for (int c = 0; c < kClasses; ++c)
@@ -66,6 +66,7 @@ struct Job {
     bool auto_release = false;
     double t_arrival = 0, t_launch = 0, t_done = 0;
     double* out_latency = nullptr;   // replay: written at completion
+    uint32_t* out_left = nullptr;    // replay: decremented at completion
 };
 
 // What the scheduler needs from the device.  All calls come from the
@@ -74,6 +75,8 @@ template <class Payload>
 struct Device {
     virtual ~Device() = default;
     virtual int lanes() const = 0;
+    virtual void on_thread_start() {}     // first call on the scheduler thread
+    virtual std::string error() const { return ""; }
     // bytes held outside the framebuffer pool (frame arenas, batch tables)
     virtual uint64_t fixed_bytes() const = 0;
     virtual int alloc(uint64_t bytes, void** p) = 0;   // S_OK or an error code
@@ -152,7 +155,7 @@ public:
     }
 
     int submit(const Payload& pl, uint64_t fb_bytes, uint64_t faces, uint64_t tiles, double arrival,
-               bool auto_release, double* out_latency, uint64_t* ticket) {
+               bool auto_release, double* out_latency, uint64_t* ticket, uint32_t* out_left = nullptr) {
         const int cls = class_of(fb_bytes);
         std::unique_lock<std::mutex> lk(mu_);
         if (cls < 0 || (cfg_.budget && dev_->fixed_bytes() + class_bytes(cls) > cfg_.budget)) {
@@ -181,6 +184,7 @@ public:
         j.auto_release = auto_release;
         j.t_arrival = arrival;
         j.out_latency = out_latency;
+        j.out_left = out_left;
         queue_.push_back(slot);
         ++stats_.submitted;
         stats_.queue_peak = std::max<uint64_t>(stats_.queue_peak, queue_.size());
@@ -217,14 +221,16 @@ public:
         }
     }
 
-    // Calls fn(job) under the lock for a completed request.
-    template <class F>
-    int with_done(uint64_t ticket, F fn) {
+    // The framebuffer block and payload of a completed request.  The block
+    // stays valid until the request is released.
+    int done_block(uint64_t ticket, Block* fb, Payload* pl) {
         std::lock_guard<std::mutex> lk(mu_);
         JobT* j = find(ticket);
         if (!j) return S_INVALID;
         if (j->state != JOB_DONE) return j->state == JOB_FAILED ? j->status : S_NOT_READY;
-        return fn(*j);
+        if (fb) *fb = j->fb;
+        if (pl) *pl = j->pl;
+        return S_OK;
     }
 
     int release(uint64_t ticket) {
@@ -254,23 +260,23 @@ public:
     // (negative for a failed request).  Returns when all are done.
     int replay(uint32_t n, const Payload* pls, const uint64_t* fb_bytes, const uint64_t* faces,
                const uint64_t* tiles, const double* arrival_s, double* latency_s) {
-        for (uint32_t i = 0; i < n; ++i) latency_s[i] = NAN;
+        for (uint32_t i = 0; i < n; ++i) latency_s[i] = -1.0;
+        uint32_t left = n;   // guarded by mu_
         const double t0 = now_s();
         int rc = S_OK;
         for (uint32_t i = 0; i < n; ++i) {
             const double t = t0 + arrival_s[i];
             while (now_s() < t) std::this_thread::yield();
-            int r = submit(pls[i], fb_bytes[i], faces[i], tiles[i], t, true, &latency_s[i], nullptr);
-            if (r != S_OK) { latency_s[i] = -1.0; rc = r; }
+            int r = submit(pls[i], fb_bytes[i], faces[i], tiles[i], t, true, &latency_s[i], nullptr, &left);
+            if (r != S_OK) {
+                rc = r;
+                std::lock_guard<std::mutex> lk(mu_);
+                --left;
+            }
         }
         std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [&] {
-            if (!running_) return true;
-            for (uint32_t i = 0; i < n; ++i)
-                if (std::isnan(latency_s[i])) return false;
-            return true;
-        });
-        return rc;
+        done_cv_.wait(lk, [&] { return left == 0 || !running_; });
+        return left == 0 ? rc : S_NOT_READY;
     }
 
     // One scheduler iteration (harvest finished lanes, then admit onto free
@@ -372,6 +378,7 @@ private:
         j.state = status == S_OK ? JOB_DONE : JOB_FAILED;
         if (status == S_OK) ++stats_.completed; else ++stats_.failed;
         if (j.out_latency) *j.out_latency = status == S_OK ? t - j.t_arrival : -1.0;
+        if (j.out_left) --*j.out_left;
         if (j.auto_release) free_job(slot);
     }
 
@@ -404,7 +411,7 @@ private:
             }
             const int r = dev_->launch(l, batch);
             if (r != S_OK) {
-                err_ = "batch launch failed";
+                err_ = "batch launch failed: " + dev_->error();
                 for (uint32_t slot : lane.slots) finish(slot, r, t);
                 lane.slots.clear();
                 done_cv_.notify_all();
@@ -424,6 +431,7 @@ private:
     }
 
     void run() {
+        dev_->on_thread_start();
         std::unique_lock<std::mutex> lk(mu_);
         while (running_) {
             bool changed = harvest();

@@ -93,3 +93,45 @@ class GpuBatch:
         self.batch.close()
         self.batch = R.Batch(self.ctx, self.fbs, [self.meshes[vs.mesh] for vs in self.wl.views], self.params)
         self.batch.set_clear(BLACK)
+
+
+class GpuServeScene:
+    """A ServeWorkload made resident: one texture set, one material, one
+    mesh per workload mesh.  `request(vs)` builds the server request for a
+    ViewSpec."""
+
+    def __init__(self, ctx, wl):
+        from .serve import request
+        self._request = request
+        self.ctx = ctx
+        self.wl = wl
+        self.textures = {k: R.Texture(ctx, t.rgba, t.filter) for k, t in wl.textures.items()}
+        m = wl.material
+        g = lambda n: self.textures[n] if n else None  # noqa: E731
+        mat = R.Material(g(m.diffuse), g(m.metallic), g(m.bump), g(m.roughness), g(m.ao), m.specular_exponent)
+        self.meshes = [R.Mesh(ctx, v, f, [R.MaterialGroup(0, 0, f.shape[0])], [mat]) for v, f in wl.meshes]
+
+    def params(self, vs):
+        return params_of(self.wl.scene(vs))
+
+    def request(self, vs):
+        return self._request(self.meshes[vs.mesh], vs.width, vs.height, self.params(vs))
+
+    def draw_plain(self, vs, with_ids=False):
+        """The same request through the plain per-frame path (clear +
+        drawMesh into a new framebuffer); returns (bgra, depth, ids)."""
+        fb = R.Framebuffer(self.ctx, vs.width, vs.height, with_ids=with_ids)
+        try:
+            fb.clear(BLACK)
+            p = self.params(vs)
+            self.ctx.check(self.ctx._lib.axr_draw(self.ctx.handle, fb.handle, self.meshes[vs.mesh].handle,
+                                                  C.byref(p)))
+            return fb.read()
+        finally:
+            fb.close()
+
+    def close(self):
+        for m in self.meshes:
+            m.close()
+        for t in self.textures.values():
+            t.close()

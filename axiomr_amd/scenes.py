@@ -255,3 +255,66 @@ def batch_workload(nviews=256, nmeshes=32, fmin=5_000, fmax=500_000, wmin=256, w
     return BatchWorkload(meshes=meshes, textures=textures,
                          material=MaterialData(diffuse="albedo", bump="normal", specular_exponent=0.64),
                          shader={"kind": 4, "lights": POINT_LIGHTS}, views=views)
+
+
+# resolution mix of the served workload: (width, height, share)
+SERVE_RESOLUTIONS = ((512, 512, 0.50), (1280, 720, 0.35), (1920, 1080, 0.15))
+
+
+@dataclass
+class ServeWorkload:
+    """Resident meshes sharing one material, plus a request generator."""
+    meshes: list                  # [(vertices (N,14), faces (F,3))]
+    textures: dict
+    material: MaterialData
+    shader: dict
+    seed: int
+
+    def requests(self, n: int, seed: int = 0, resolutions=SERVE_RESOLUTIONS) -> list:
+        """n requests: uniform mesh id, resolution drawn from `resolutions`
+        ((width, height, share) triples), random orbit camera."""
+        rng = np.random.default_rng([self.seed, seed])
+        shares = np.array([r[2] for r in resolutions])
+        res = rng.choice(len(resolutions), size=n, p=shares / shares.sum())
+        out = []
+        for i in range(n):
+            w, h, _ = resolutions[res[i]]
+            pos = orbit_position(rng.uniform(2.2, 4.0), rng.uniform(0.0, 360.0), rng.uniform(-60.0, 60.0))
+            out.append(ViewSpec(mesh=int(rng.integers(len(self.meshes))), width=w, height=h, camera_pos=pos))
+        return out
+
+    def scene(self, vs: ViewSpec) -> SceneData:
+        v, f = self.meshes[vs.mesh]
+        return SceneData(name=f"mesh{vs.mesh}_{vs.width}x{vs.height}", vertices=v, faces=f, width=vs.width,
+                         height=vs.height, camera_pos=vs.camera_pos, camera_target=vs.camera_target, fov=vs.fov,
+                         shader=self.shader, groups=[(0, 0, f.shape[0])], materials=[self.material],
+                         textures=self.textures)
+
+    @property
+    def total_faces(self) -> int:
+        return sum(f.shape[0] for _, f in self.meshes)
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
+    """Arrival offsets (seconds) of a Poisson stream of `rate` per second."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def serve_workload(nmeshes=64, fmin=10_000, fmax=2_000_000, tex=2048, seed=29, threads=8) -> ServeWorkload:
+    """Resident meshes with log-uniform face counts (icospheres, 20 n^2
+    faces), normal-mapped Phong with 3 point lights over the noise albedo
+    and normal map of the 1080p icosphere scene."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    ns = []
+    for _ in range(nmeshes):
+        F = math.exp(rng.uniform(math.log(fmin), math.log(fmax)))
+        ns.append(max(2, int(round(math.sqrt(F / 20.0)))))
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        meshes = list(ex.map(lambda a: icosphere(a[1], 0.1, 1000 + a[0]), enumerate(ns)))
+    textures = {"albedo": TextureData(noise_texture(tex, tex, 5), 1),
+                "normal": TextureData(normal_map(tex, tex, 0.7, 6), 1)}
+    return ServeWorkload(meshes=meshes, textures=textures,
+                         material=MaterialData(diffuse="albedo", bump="normal", specular_exponent=0.64),
+                         shader={"kind": 4, "lights": POINT_LIGHTS}, seed=seed)

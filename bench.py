@@ -173,6 +173,99 @@ def run_batch(wl, reps, tile, use_tuned):
                 faces=wl.total_faces, pixels=wl.total_pixels, config=cfg, plain_config=plain_cfg)
 
 
+SERVE_RATES = (100, 250, 500, 1000, 2500, 5000)
+
+
+def run_serve(wl, rates, seconds_per_rate, lanes, budget, tile, use_tuned, burst=2000):
+    """The served form: an open-loop Poisson stream at each rate, replayed
+    natively (submission at the scheduled instants, results released on
+    completion); p50/p99 latency = completion - scheduled arrival.  Then a
+    burst (every request at t=0) for the capacity in frames/s, and the same
+    burst through the plain per-frame path (clear + drawMesh per request,
+    queued on one stream) as the baseline."""
+    import axiomr_amd as axr
+    from axiomr_amd import resident as H
+    from axiomr_amd import scenes as S
+    from axiomr_amd.serve import Server
+
+    ctx = axr.Context(0)
+    cfg = apply_config(ctx, "serve", tile, use_tuned) if use_tuned and _has_tuned("serve") else \
+        apply_config(ctx, "plain", tile, use_tuned)
+    gs = H.GpuServeScene(ctx, wl)
+    faces = [f.shape[0] for _, f in wl.meshes]
+
+    def build(views):
+        return [gs.request(v) for v in views]
+
+    srv = Server(ctx, lanes=lanes, max_batch_views=64, hbm_budget=budget)
+    srv.replay(build(wl.requests(300, seed=1)), np.zeros(300))      # warm the pool and the lanes
+    out = []
+    for rate in rates:
+        n = int(min(4000, max(200, rate * seconds_per_rate)))
+        views = wl.requests(n, seed=rate)
+        reqs = build(views)
+        arr = S.poisson_arrivals(n, rate, seed=rate)
+        st0 = srv.stats()
+        lat = srv.replay(reqs, arr)
+        st1 = srv.stats()
+        span = float(np.max(arr + lat))
+        nb = st1["batches"] - st0["batches"]
+        rec = {"rate": rate, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+               "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+               "max_ms": round(float(lat.max()) * 1e3, 3),
+               "frames_per_s": round(n / span, 1),
+               "gtri_per_s": round(sum(faces[v.mesh] for v in views) / span / 1e9, 4),
+               "mean_views_per_batch": round((st1["batched_views"] - st0["batched_views"]) / max(1, nb), 2),
+               "overlapped_launches": st1["launches_overlapped"] - st0["launches_overlapped"]}
+        log(f"[bench] serve {rate} req/s: {rec}")
+        out.append(rec)
+    # capacity: a burst of `burst` requests at t=0
+    views = wl.requests(burst, seed=7)
+    reqs = build(views)
+    t0 = time.perf_counter()
+    lat = srv.replay(reqs, np.zeros(burst))
+    cap_s = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    # plain per-frame path on the same burst
+    import ctypes as C
+    from axiomr_amd import render as R
+    fbs = {}
+    for v in views:
+        if (v.width, v.height) not in fbs:
+            fbs[(v.width, v.height)] = R.Framebuffer(ctx, v.width, v.height)
+    L = ctx._lib
+    bgra = H.BLACK.packed_bgra()
+    plist = [gs.params(v) for v in views]
+    for p in plist:
+        p.flags = 1   # AXR_DRAW_ASYNC
+
+    def plain_all():
+        for v, p in zip(views, plist):
+            fb = fbs[(v.width, v.height)]
+            L.axr_fb_clear(fb.handle, bgra, float("inf"))
+            L.axr_draw(ctx.handle, fb.handle, gs.meshes[v.mesh].handle, C.byref(p))
+        ctx.sync()
+
+    plain_all()
+    t0 = time.perf_counter()
+    plain_all()
+    plain_s = time.perf_counter() - t0
+    for fb in fbs.values():
+        fb.close()
+    gs.close()
+    ctx.close()
+    burst_faces = sum(faces[v.mesh] for v in views)
+    return dict(rates=out, capacity_frames_per_s=burst / cap_s, capacity_gtri_per_s=burst_faces / cap_s / 1e9,
+                burst_p50_ms=float(np.percentile(lat, 50)) * 1e3, plain_frames_per_s=burst / plain_s,
+                speedup=plain_s / cap_s, stats=st, config=cfg)
+
+
+def _has_tuned(workload):
+    from axiomr_amd import autotune as AT
+    return AT.load(workload) is not None
+
+
 def cpu_baseline(scene, budget_s=12.0, threads=16):
     """The CPU checker (restated reference tiled pipeline with its thread
     pool) on the same frame, repeated until ~budget_s of CPU work."""
@@ -241,6 +334,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-form leg")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="stream length per arrival rate")
+    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain-copy kernel used to scale the --pmc byte counters")
     args = ap.parse_args()
@@ -297,6 +394,30 @@ def main():
                  "batch_ms": round(b["batch_ms"], 3), "stages_ms": {k: round(v, 4) for k, v in b["stages_ms"].items()},
                  "tunables": b["config"], "plain_tunables": b["plain_config"]}
 
+    served = None
+    if not args.no_serve:
+        t = time.perf_counter()
+        swl = S.serve_workload()
+        log(f"[bench] serve workload: {len(swl.meshes)} meshes, {swl.total_faces} faces "
+            f"({time.perf_counter() - t:.1f}s)")
+        budget = int(args.serve_budget_gb * (1 << 30))
+        sv = run_serve(swl, SERVE_RATES, args.serve_seconds, args.serve_lanes, budget, args.tile, not args.untuned)
+        log(f"[bench] serve capacity {sv['capacity_frames_per_s']:.0f} frames/s, plain "
+            f"{sv['plain_frames_per_s']:.0f} frames/s, x{sv['speedup']:.2f}")
+        st = sv["stats"]
+        served = {"workload": f"Poisson request stream, {len(swl.meshes)} resident displaced icospheres "
+                              "(10k-2M triangles, log-uniform), 50% 512^2 / 35% 1280x720 / 15% 1920x1080, "
+                              "random orbit cameras, Phong + textures, 3 point lights",
+                  "lanes": args.serve_lanes, "hbm_budget_bytes": budget, "rates": sv["rates"],
+                  "capacity_frames_per_s": round(sv["capacity_frames_per_s"], 1),
+                  "capacity_gtri_per_s": round(sv["capacity_gtri_per_s"], 3),
+                  "plain_frames_per_s": round(sv["plain_frames_per_s"], 1),
+                  "speedup_vs_plain": round(sv["speedup"], 3),
+                  "fb_peak_bytes": st["fb_peak_bytes"], "fixed_bytes": st["fixed_bytes"],
+                  "batches": st["batches"], "max_views_in_batch": st["max_views_in_batch"],
+                  "max_lanes_busy": st["max_lanes_busy"], "overflow_retries": st["overflow_retries"],
+                  "tunables": sv["config"]}
+
     cpu = None
     if not args.no_cpu:
         c = cpu_baseline(scene, args.cpu_budget)
@@ -332,6 +453,7 @@ def main():
                      "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 5)},
         "cpu_baseline": cpu,
         "batched": batch,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

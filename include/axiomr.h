@@ -175,6 +175,79 @@ int axr_batch_set_clear(axr_batch* batch, int enable, uint32_t bgra, float depth
 int axr_batch_launch(axr_batch* batch);    /* enqueue (graph replay); completes at axr_sync */
 int axr_batch_destroy(axr_batch* batch);
 
+/* ---- served form: continuous batching of render requests --------------- */
+/* No counterpart in the reference (it renders one window's frames).  Each
+ * request is one drawMesh (src/tiled_pipeline.cpp:143-322) of a resident
+ * mesh into a framebuffer of the requested size, cleared to clear_bgra and
+ * +inf depth first (src/framebuffer.cpp:6-14).  Requests are admitted in
+ * arrival order as lanes and HBM free up, and drawn in batches on `lanes`
+ * streams, each with its own frame arena, so batches overlap on the device.
+ * A request's pixels do not depend on what else shares its batch.  Meshes,
+ * textures and the context must outlive the requests that use them. */
+typedef struct axr_server_config {
+    int lanes;                  /* batches in flight at once, 1..8 (0: 3) */
+    uint32_t max_batch_views;   /* requests per batch launch (0: 64) */
+    uint64_t max_batch_faces;   /* submitted triangles per batch; a larger
+                                   single request runs alone (0: 8 Mi) */
+    uint64_t max_batch_tiles;   /* bins per batch, and per request (0: 1 Mi) */
+    uint64_t hbm_budget;        /* bytes for frame arenas + framebuffers;
+                                   0 = no limit beyond the context's */
+    uint32_t clear_bgra;        /* clear colour of every request */
+    int with_ids;               /* keep each request's winner-key plane */
+} axr_server_config;
+
+typedef struct axr_serve_request {
+    const axr_mesh* mesh;
+    int width, height;
+    axr_draw_params params;     /* flags ignored */
+} axr_serve_request;
+
+typedef enum axr_request_state {
+    AXR_REQ_QUEUED = 1,         /* waiting for a lane or for HBM */
+    AXR_REQ_RUNNING = 2,
+    AXR_REQ_DONE = 3,
+    AXR_REQ_FAILED = 4
+} axr_request_state;
+
+typedef struct axr_server_stats {
+    uint64_t submitted, completed, failed, rejected;
+    uint64_t batches, batched_views, max_views_in_batch;
+    uint64_t launches_overlapped;   /* batches launched while another lane ran */
+    uint32_t max_lanes_busy;
+    uint32_t reserved0;
+    uint64_t deferred;              /* admission passes stopped by the budget */
+    uint64_t fb_reserved_bytes, fb_peak_bytes, fb_in_use_bytes;
+    uint64_t fixed_bytes;           /* frame arenas and batch tables */
+    uint64_t queue_peak;
+    uint64_t overflow_retries;      /* batches redrawn after an arena overflow */
+} axr_server_stats;
+
+/* Starts the scheduler thread.  The context's tunables (axr_config) at this
+ * call are the server's.  Fails with AXR_ERR_BUDGET when the lanes' frame
+ * arenas alone exceed hbm_budget. */
+int axr_server_create(axr_ctx* ctx, const axr_server_config* cfg, axr_server** out);
+/* Drains running batches, then frees everything (unreleased results too). */
+int axr_server_destroy(axr_server* server);
+const char* axr_server_last_error(axr_server* server);
+/* Queues one request.  AXR_ERR_BUDGET if its framebuffer can never fit the
+ * budget next to the arenas; a request that fits later waits in the queue. */
+int axr_server_submit(axr_server* server, const axr_serve_request* req, uint64_t* ticket);
+/* *state = axr_request_state; returns the request's error once FAILED. */
+int axr_server_poll(axr_server* server, uint64_t ticket, int* state);
+/* AXR_OK when done, AXR_ERR_NOT_READY on timeout, else the request's error. */
+int axr_server_wait(axr_server* server, uint64_t ticket, double timeout_s);
+/* Copies a done request's planes to the host (any pointer may be null). */
+int axr_server_read(axr_server* server, uint64_t ticket, uint8_t* bgra, float* depth, uint32_t* ids);
+/* Returns the request's framebuffer to the pool; the ticket becomes invalid. */
+int axr_server_release(axr_server* server, uint64_t ticket);
+int axr_server_get_stats(axr_server* server, axr_server_stats* out);
+/* Open-loop driver: request i is submitted at (call time + arrival_s[i]),
+ * non-decreasing; results are released as they complete and latency_s[i]
+ * receives completion minus scheduled arrival in seconds (-1 if failed).
+ * Returns after the last request completes. */
+int axr_server_replay(axr_server* server, uint32_t n, const axr_serve_request* reqs, const double* arrival_s,
+                      double* latency_s);
+
 /* ---- synthetic scenes (host only; no device work) ---------------------- */
 /* UV sphere of radius r with `stacks` x `slices` quads: (stacks+1)*(slices+1)
  * vertices, 2*stacks*slices faces. */
