@@ -56,7 +56,7 @@ class AxrDrawParams(C.Structure):
 
 class AxrConfig(C.Structure):
     _fields_ = [("tile_size", C.c_int32), ("raster_threads", C.c_int32), ("bin_capacity", C.c_int32),
-                ("small_footprint", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("small_footprint", C.c_int32), ("raster_mode", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class AxrKernelStats(C.Structure):
@@ -87,6 +87,7 @@ class AxrServerStats(C.Structure):
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 1, 2, 3, 4
+RASTER_BINNED, RASTER_VISIBILITY = 0, 1
 
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
@@ -129,6 +130,7 @@ SIGNATURES = {
     "axr_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double]),
     "axr_server_read": (C.c_int, [_P, C.c_uint64, _U8, _F, _U32]),
     "axr_server_release": (C.c_int, [_P, C.c_uint64]),
+    "axr_server_pause": (C.c_int, [_P, C.c_int]),
     "axr_server_get_stats": (C.c_int, [_P, C.POINTER(AxrServerStats)]),
     "axr_server_replay": (C.c_int, [_P, C.c_uint32, C.POINTER(AxrServeRequest), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),

@@ -1,13 +1,14 @@
-"""Autotuner for the binning pipeline's tunables (axr_config): bin edge
-(8/16/32 px), threads per bin workgroup (64/128/256), per-wave bin-pair
-capacity (64..1024) and the one-lane footprint threshold.  None of them
-changes a pixel; they only move time between the setup, scatter and raster
-kernels.
+"""Autotuner for the render path's tunables (axr_config): raster mode
+(binned / visibility buffer), bin edge (8/16/32 px), threads per bin
+workgroup (64/128/256), per-wave bin-pair capacity (64..1024, binned mode)
+and the one-lane footprint threshold.  None of them changes a pixel; they
+only move time between the kernels.
 
-Coordinate descent: bin edge x workgroup size first (9 points), then the
-bin capacity, then the footprint threshold, each point timed as the median
-of a few synchronised runs of the caller's frame function.  Results are
-cached in a JSON file keyed by device and workload name.
+Coordinate descent: raster mode x bin edge x workgroup size first (18
+points), then the bin capacity (binned mode only), then the footprint
+threshold, each point timed as the median of a few synchronised runs of the
+caller's frame function.  Results are cached in a JSON file keyed by device
+and workload name.
 """
 from __future__ import annotations
 
@@ -16,6 +17,7 @@ import time
 from pathlib import Path
 from typing import Callable, Dict, Optional
 
+MODES = (0, 1)          # AXR_RASTER_BINNED, AXR_RASTER_VISIBILITY
 TILES = (8, 16, 32)
 THREADS = (64, 128, 256)
 CAPACITIES = (64, 128, 256, 512, 1024)
@@ -55,20 +57,23 @@ def tune(ctx, make_frame: Callable[[], Callable[[], None]], log: Optional[Callab
         return t
 
     best_t = run(best)
-    for tile in TILES:
-        for nt in THREADS:
-            cfg = {**best, "tile_size": tile, "raster_threads": nt}
-            if cfg == best:
-                continue
-            t = run(cfg)
-            if t < best_t:
-                best, best_t = cfg, t
-    for cap in CAPACITIES:
-        cfg = {**best, "bin_capacity": cap}
-        if cfg != best:
-            t = run(cfg)
-            if t < best_t:
-                best, best_t = cfg, t
+    start = dict(best)
+    for mode in MODES:
+        for tile in TILES:
+            for nt in THREADS:
+                cfg = {**start, "raster_mode": mode, "tile_size": tile, "raster_threads": nt}
+                if cfg == start:
+                    continue
+                t = run(cfg)
+                if t < best_t:
+                    best, best_t = cfg, t
+    if best.get("raster_mode", 0) == 0:
+        for cap in CAPACITIES:
+            cfg = {**best, "bin_capacity": cap}
+            if cfg != best:
+                t = run(cfg)
+                if t < best_t:
+                    best, best_t = cfg, t
     for fp in FOOTPRINTS:
         cfg = {**best, "small_footprint": fp}
         if cfg != best:

@@ -79,6 +79,7 @@ struct ViewDesc {
     const uint32_t* face_colors;   // RGBA8 per face or null
     const DevGroup* groups;
     const DevMaterial* materials;
+    DevGroup g0;                   // groups[0], for single-group meshes without a load
     uint32_t ngroups;
     uint32_t nfaces;               // faces in submission order (sum of group counts)
     // transform
@@ -93,6 +94,8 @@ struct ViewDesc {
     // offsets of this view inside the batch-wide arrays
     uint32_t face_base;            // first slot of this view in the record array
     uint32_t tile_base;            // first bin of this view
+    uint32_t pix_base;             // first pixel of this view in the visibility buffer
+    uint32_t pad;
 };
 
 // Triangle record, 48 bytes (three 16-byte loads).
@@ -123,11 +126,24 @@ enum Counter : uint32_t {
     CNT_NEED_CLIP_FACES = 5,
     CNT_NEED_CLIP_RECS = 6,
     CNT_NEED_PAIRS = 7,
+    CNT_BIG_ITEMS = 8,    // visibility mode: work items of large triangles; with
+    CNT_BIG_RECS = 9,     // the next word one 64-bit counter (records << 32 | items)
+    CNT_NEED_BIG_RECS = 10,
+    CNT_NEED_BIG_ITEMS = 11,
     CNT_COUNT = 16,
 };
 constexpr uint32_t ERR_CLIP_FACES = 1u;
 constexpr uint32_t ERR_CLIP_RECS = 2u;
 constexpr uint32_t ERR_PAIRS = 4u;
+constexpr uint32_t ERR_BIG = 8u;
+
+// Raster modes (axr_config.raster_mode).
+enum RasterMode : int32_t {
+    RASTER_BINNED = 0,       // setup -> bins -> per-bin LDS raster + resolve
+    RASTER_VISIBILITY = 1,   // setup rasterizes small triangles into a per-pixel
+                             // 64-bit key buffer; large ones go through work
+                             // items; a per-bin pass resolves and shades
+};
 
 // Device frame arena: the per-frame working memory of the binning pipeline.
 struct Arena {
@@ -152,6 +168,15 @@ struct Arena {
                                // a single atomic)
     uint32_t cap_ovf;          // overflow pool entries (after the regions)
     uint32_t cap_bins;
+    // visibility mode
+    unsigned long long* vkeys; // per pixel of every view: (ordered depth << 32 | key),
+                               // all ones when empty; the resolve leaves it empty
+    TriRec* big_rec;           // large triangles (and clipped pieces) as records
+    uint32_t* big_start;       // first work item of each large-triangle record
+    uint32_t cap_vkeys;
+    uint32_t cap_big_recs;
+    uint32_t cap_big_items;
+    uint32_t pad;
 };
 
 // Kernel arguments common to every stage.  `table` points to `nviews`
@@ -169,6 +194,8 @@ struct KArgs {
     int32_t raster_threads;     // workgroup size of k_raster: 64, 128 or 256
     int32_t small_footprint;    // a triangle covering at most this many pixels
                                 // of a bin is walked by one lane
+    int32_t mode;               // RasterMode
+    uint32_t total_pixels;
     ViewDesc one;
 };
 

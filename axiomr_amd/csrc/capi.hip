@@ -22,7 +22,7 @@ struct axr_batch {
     bool clear = false;            // clear every view's framebuffer before drawing
     uint32_t clear_bgra = 0;
     float clear_depth = 0.0f;
-    uint32_t wave_cap = 128;       // bin capacity the batch was built with
+    FrameSize fs;                  // arena needs, with the tunables the batch was built with
     KArgs ka{};
     size_t bytes = 0;
     hipGraph_t graph = nullptr;
@@ -115,19 +115,35 @@ static int grow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, DevBuf& b, size_t by
     return AXR_OK;
 }
 
-int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uint64_t tiles, uint32_t wave_cap,
-                 uint64_t need_clip_faces, uint64_t need_clip_recs, uint64_t need_pairs) {
-    if (faces > 0x7FFFFFFFull || tiles > 0x7FFFFFFFull)
-        return fail(err, AXR_ERR_INVALID, "draw too large (%llu faces, %llu bins)", (unsigned long long)faces,
-                    (unsigned long long)tiles);
+FrameSize frame_size(const KArgs& ka, uint32_t wave_cap) {
+    FrameSize fs;
+    fs.faces = ka.total_faces;
+    fs.tiles = ka.total_tiles;
+    fs.pixels = ka.mode == RASTER_VISIBILITY ? ka.total_pixels : 0;
+    fs.wave_cap = wave_cap;
+    return fs;
+}
+
+int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs, const ArenaNeeds& need) {
+    const uint64_t faces = fs.faces, tiles = fs.tiles;
+    const uint32_t wave_cap = fs.wave_cap;
+    if (faces > 0x7FFFFFFFull || tiles > 0x7FFFFFFFull || fs.pixels > 0xFFFFFFFFull)
+        return fail(err, AXR_ERR_INVALID, "draw too large (%llu faces, %llu bins, %llu pixels)",
+                    (unsigned long long)faces, (unsigned long long)tiles, (unsigned long long)fs.pixels);
     Arena& a = fa.a;
-    uint64_t clip_faces = std::max<uint64_t>({4096, faces / 16, need_clip_faces, a.cap_clip_faces});
-    uint64_t clip_recs = std::max<uint64_t>({2 * clip_faces, need_clip_recs, a.cap_clip_recs});
+    uint64_t clip_faces = std::max<uint64_t>({4096, faces / 16, need.clip_faces, a.cap_clip_faces});
+    uint64_t clip_recs = std::max<uint64_t>({2 * clip_faces, need.clip_recs, a.cap_clip_recs});
     // one pair region per setup wave (setup runs 4 waves per 256 faces)
     uint64_t waves = std::max<uint64_t>(4 * ((faces + 255) / 256), a.cap_waves);
-    uint64_t ovf = std::max<uint64_t>({faces / 4 + 16 * tiles + 4096, need_pairs, a.cap_ovf});
-    uint64_t nbins = std::max<uint64_t>({2 * faces + 16 * tiles + 4096, need_pairs, a.cap_bins});
+    uint64_t ovf = std::max<uint64_t>({faces / 4 + 16 * tiles + 4096, need.pairs, a.cap_ovf});
+    uint64_t nbins = std::max<uint64_t>({2 * faces + 16 * tiles + 4096, need.pairs, a.cap_bins});
     uint64_t scan_blocks = (tiles + 4095) / 4096 + 1;
+    // visibility mode: large triangles and their 256-pixel work items
+    uint64_t big_recs = 0, big_items = 0;
+    if (fs.pixels) {
+        big_recs = std::max<uint64_t>({4096, faces / 8, need.big_recs, a.cap_big_recs});
+        big_items = std::max<uint64_t>({16384, faces / 4 + fs.pixels / 32, need.big_items, a.cap_big_items});
+    }
     const uint64_t gen_start = fa.gen;
     int r;
     if ((r = grow(ctx, err, fa, fa.counters, CNT_COUNT * 4)) ||
@@ -141,7 +157,10 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uin
         (r = grow(ctx, err, fa, fa.wave_pairs, waves * 4)) ||
         (r = grow(ctx, err, fa, fa.bins, nbins * 4)) ||
         (r = grow(ctx, err, fa, fa.active, (tiles + 1) * 4)) ||
-        (r = grow(ctx, err, fa, fa.scan_sums, scan_blocks * 8)))
+        (r = grow(ctx, err, fa, fa.scan_sums, scan_blocks * 8)) ||
+        (r = grow(ctx, err, fa, fa.vkeys, fs.pixels * 8)) ||
+        (r = grow(ctx, err, fa, fa.big_rec, big_recs * sizeof(TriRec))) ||
+        (r = grow(ctx, err, fa, fa.big_start, big_recs * 4)))
         return r;
     a.counters = (uint32_t*)fa.counters.p;
     a.rec = (TriRec*)fa.rec.p;
@@ -155,6 +174,9 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uin
     a.bins = (uint32_t*)fa.bins.p;
     a.active = (uint32_t*)fa.active.p;
     a.scan_sums = (unsigned long long*)fa.scan_sums.p;
+    a.vkeys = (unsigned long long*)fa.vkeys.p;
+    a.big_rec = (TriRec*)fa.big_rec.p;
+    a.big_start = (uint32_t*)fa.big_start.p;
     a.cap_clip_faces = (uint32_t)std::min<uint64_t>(fa.clip_faces.bytes / 4, 0xFFFFFFFFull);
     a.cap_clip_recs = (uint32_t)std::min<uint64_t>(
         {fa.clip_rec.bytes / sizeof(TriRec), fa.clip_verts.bytes / (3 * sizeof(Vertex56)), 0x7FFFFFFFull});
@@ -164,6 +186,11 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uin
         {fa.wave_pairs.bytes / 4, 0x7FFFFFFFull / wave_cap, fa.pairs.bytes / 16 / wave_cap});
     a.cap_ovf = (uint32_t)std::min<uint64_t>(fa.pairs.bytes / 16 - (uint64_t)a.cap_waves * wave_cap, 0x7FFFFFFFull);
     a.cap_bins = (uint32_t)std::min<uint64_t>(fa.bins.bytes / 4, 0x7FFFFFFFull);
+    a.cap_vkeys = (uint32_t)std::min<uint64_t>(fa.vkeys.bytes / 8, 0xFFFFFFFFull);
+    a.cap_big_recs = (uint32_t)std::min<uint64_t>(
+        {fa.big_rec.bytes / sizeof(TriRec), fa.big_start.bytes / 4, 0x7FFFFFFFull});
+    // items are only counted (no storage): the cap bounds the work of k_big
+    a.cap_big_items = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(big_items, a.cap_big_items), 0x7FFFFFFFull);
     if (fa.gen != gen_start) fa.dirty = true;
     return AXR_OK;
 }
@@ -183,14 +210,33 @@ int arena_create(axr_ctx* ctx, ErrSink& err, FrameArena& fa) {
     hipError_t e = hipStreamCreateWithFlags(&fa.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&fa.host_counters, CNT_COUNT * 4, hipHostMallocDefault);
     if (e != hipSuccess) return hip_fail(err, e, "frame arena stream");
-    int r = arena_ensure(ctx, err, fa, 1024, 1024, (uint32_t)ctx->cfg.bin_capacity);
+    FrameSize fs;
+    fs.faces = 1024;
+    fs.tiles = 1024;
+    fs.wave_cap = (uint32_t)ctx->cfg.bin_capacity;
+    int r = arena_ensure(ctx, err, fa, fs);
     return r ? r : arena_reset(err, fa);
+}
+
+static void arena_buffers(FrameArena& fa, DevBuf* (&out)[15]) {
+    DevBuf* bufs[15] = {&fa.counters, &fa.rec,       &fa.clip_faces, &fa.clip_rec, &fa.clip_verts,
+                        &fa.tile_count, &fa.tile_off, &fa.pairs,     &fa.wave_pairs, &fa.bins,
+                        &fa.active,   &fa.scan_sums,  &fa.vkeys,     &fa.big_rec,  &fa.big_start};
+    for (int i = 0; i < 15; ++i) out[i] = bufs[i];
+}
+
+uint64_t arena_bytes(const FrameArena& fa) {
+    DevBuf* bufs[15];
+    arena_buffers(const_cast<FrameArena&>(fa), bufs);
+    uint64_t b = 0;
+    for (DevBuf* d : bufs) b += d->bytes;
+    return b;
 }
 
 void arena_destroy(axr_ctx* ctx, FrameArena& fa) {
     if (fa.stream) (void)hipStreamSynchronize(fa.stream);
-    DevBuf* bufs[] = {&fa.counters, &fa.rec, &fa.clip_faces, &fa.clip_rec, &fa.clip_verts, &fa.tile_count,
-                      &fa.tile_off, &fa.pairs, &fa.wave_pairs, &fa.bins, &fa.active, &fa.scan_sums};
+    DevBuf* bufs[15];
+    arena_buffers(fa, bufs);
     for (DevBuf* b : bufs) free_buf(ctx, *b);
     if (fa.host_counters) (void)hipHostFree(fa.host_counters);
     if (fa.stream) (void)hipStreamDestroy(fa.stream);
@@ -219,18 +265,23 @@ int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev) 
     return e == hipSuccess ? AXR_OK : hip_fail(err, e, "kernel launch");
 }
 
-int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uint64_t tiles,
-                         uint32_t wave_cap) {
+int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs) {
     const uint32_t* c = fa.host_counters;
     if (!c[CNT_ERROR]) return AXR_OK;
-    uint64_t nf = c[CNT_NEED_CLIP_FACES], nr = c[CNT_NEED_CLIP_RECS], np = c[CNT_NEED_PAIRS];
-    int r = arena_ensure(ctx, err, fa, faces, tiles, wave_cap, nf ? nf + nf / 2 : 0, nr ? nr + nr / 2 : 0,
-                         np ? np + np / 4 : 0);
+    auto grown = [](uint64_t n, uint64_t num, uint64_t den) { return n ? n + n * num / den : 0; };
+    ArenaNeeds need;
+    need.clip_faces = grown(c[CNT_NEED_CLIP_FACES], 1, 2);
+    need.clip_recs = grown(c[CNT_NEED_CLIP_RECS], 1, 2);
+    need.pairs = grown(c[CNT_NEED_PAIRS], 1, 4);
+    need.big_recs = grown(c[CNT_NEED_BIG_RECS], 1, 2);
+    need.big_items = grown(c[CNT_NEED_BIG_ITEMS], 1, 2);
+    int r = arena_ensure(ctx, err, fa, fs, need);
     if (r) return r;
     if ((r = arena_reset(err, fa))) return r;
-    return fail(err, AXR_ERR_OVERFLOW, "frame arena overflow (clip faces %llu, clip records %llu, bin pairs %llu); "
-                "arena grown, frames since the last sync were not drawn",
-                (unsigned long long)nf, (unsigned long long)nr, (unsigned long long)np);
+    return fail(err, AXR_ERR_OVERFLOW, "frame arena overflow (clip faces %u, clip records %u, bin pairs %u, "
+                "large triangles %u, large-triangle items %u); arena grown, frames since the last sync were not drawn",
+                c[CNT_NEED_CLIP_FACES], c[CNT_NEED_CLIP_RECS], c[CNT_NEED_PAIRS], c[CNT_NEED_BIG_RECS],
+                c[CNT_NEED_BIG_ITEMS]);
 }
 
 ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,
@@ -241,6 +292,7 @@ ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* i
     v.idx = m->idx;
     v.face_colors = m->face_colors;
     v.groups = m->groups;
+    v.g0 = m->g0;
     v.materials = m->materials;
     v.ngroups = m->ngroups;
     v.nfaces = m->nfaces_sub;
@@ -367,6 +419,9 @@ int axr_ctx_set_config(axr_ctx* ctx, const axr_config* cfg) {
         return fail(ctx->err, AXR_ERR_INVALID, "bin_capacity must be in [64, 1024] (got %d)", cfg->bin_capacity);
     if (cfg->small_footprint < 1 || cfg->small_footprint > 1024)
         return fail(ctx->err, AXR_ERR_INVALID, "small_footprint must be in [1, 1024] (got %d)", cfg->small_footprint);
+    if (cfg->raster_mode != AXR_RASTER_BINNED && cfg->raster_mode != AXR_RASTER_VISIBILITY)
+        return fail(ctx->err, AXR_ERR_INVALID, "raster_mode must be AXR_RASTER_BINNED or AXR_RASTER_VISIBILITY (got %d)",
+                    cfg->raster_mode);
     ctx->cfg = *cfg;
     return AXR_OK;
 }
@@ -417,9 +472,12 @@ int axr_sync(axr_ctx* ctx) {
     if (r) return r;
     if (ctx->main.host_counters[CNT_ERROR]) {
         // grow to what the kernels reported; keep the current capacity as the floor
-        uint64_t faces = ctx->main.rec.bytes / sizeof(TriRec);
-        uint64_t tiles = ctx->main.tile_count.bytes / 4 - 1;
-        return arena_check_overflow(ctx, ctx->err, ctx->main, faces, tiles, (uint32_t)ctx->cfg.bin_capacity);
+        FrameSize fs;
+        fs.faces = ctx->main.rec.bytes / sizeof(TriRec);
+        fs.tiles = ctx->main.tile_count.bytes / 4 - 1;
+        fs.pixels = ctx->main.vkeys.bytes / 8;
+        fs.wave_cap = ctx->main.a.wave_cap;
+        return arena_check_overflow(ctx, ctx->err, ctx->main, fs);
     }
     return AXR_OK;
 }
@@ -497,6 +555,7 @@ int axr_mesh_create(axr_ctx* ctx, const float* verts56, uint64_t nverts, const u
     m->ngroups = ngroups;
     m->nmaterials = nmaterials;
     m->nfaces_sub = (uint32_t)rank;
+    m->g0 = dg[0];
     std::vector<float> pos(3 * nverts);
     for (uint64_t i = 0; i < nverts; ++i)
         for (int k = 0; k < 3; ++k) pos[3 * i + k] = verts56[14 * i + k];
@@ -618,8 +677,10 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     ka.tile_size = tile;
     ka.raster_threads = ctx->cfg.raster_threads;
     ka.small_footprint = ctx->cfg.small_footprint;
-    const uint32_t wave_cap = (uint32_t)ctx->cfg.bin_capacity;
-    int r = arena_ensure(ctx, ctx->err, ctx->main, ka.total_faces, ka.total_tiles, wave_cap);
+    ka.mode = ctx->cfg.raster_mode;
+    ka.total_pixels = (uint32_t)((uint64_t)fb->w * (uint64_t)fb->h);
+    const FrameSize fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
+    int r = arena_ensure(ctx, ctx->err, ctx->main, fs);
     if (r) return r;
     const bool async = (p->flags & AXR_DRAW_ASYNC) != 0;
     for (int attempt = 0; attempt < 8; ++attempt) {
@@ -627,7 +688,7 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
         if (async) return AXR_OK;
         if ((r = arena_read_counters(ctx->err, ctx->main))) return r;
         if (!ctx->main.host_counters[CNT_ERROR]) return AXR_OK;
-        r = arena_check_overflow(ctx, ctx->err, ctx->main, ka.total_faces, ka.total_tiles, wave_cap);
+        r = arena_check_overflow(ctx, ctx->err, ctx->main, fs);
         if (r != AXR_ERR_OVERFLOW) return r;
         // the raster stage was skipped, so the framebuffer is untouched: redraw
     }
@@ -660,6 +721,7 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
         ViewDesc v = make_view(meshes[i], fb->color, fb->depth, fb->ids, fb->w, fb->h, &params[i], tile);
         v.face_base = (uint32_t)faces;
         v.tile_base = (uint32_t)tiles;
+        v.pix_base = (uint32_t)pixels;
         fbase[i] = (uint32_t)faces;
         tbase[i] = (uint32_t)tiles;
         pbase[i] = (uint32_t)pixels;
@@ -708,9 +770,11 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     ka.tile_size = tile;
     ka.raster_threads = ctx->cfg.raster_threads;
     ka.small_footprint = ctx->cfg.small_footprint;
+    ka.mode = ctx->cfg.raster_mode;
+    ka.total_pixels = (uint32_t)pixels;
     ka.one = b->views[0];
-    b->wave_cap = (uint32_t)ctx->cfg.bin_capacity;
-    if ((r = arena_ensure(ctx, err, ctx->main, faces, tiles, b->wave_cap))) { axr_batch_destroy(b); return r; }
+    b->fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
+    if ((r = arena_ensure(ctx, err, ctx->main, b->fs))) { axr_batch_destroy(b); return r; }
     *out = b;
     return AXR_OK;
 }
@@ -720,7 +784,7 @@ int axr_batch_launch(axr_batch* b) {
     axr_ctx* ctx = b->ctx;
     (void)hipSetDevice(ctx->device);
     FrameArena& fa = ctx->main;
-    int r = arena_ensure(ctx, ctx->err, fa, b->ka.total_faces, b->ka.total_tiles, b->wave_cap);
+    int r = arena_ensure(ctx, ctx->err, fa, b->fs);
     if (r) return r;
     if (ctx->profiling) {   // events need the eager path
         if (b->clear) {

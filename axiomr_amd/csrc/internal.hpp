@@ -35,7 +35,7 @@ struct ErrSink {
 struct FrameArena {
     hipStream_t stream = nullptr;
     DevBuf counters, rec, clip_faces, clip_rec, clip_verts, tile_count, tile_off, pairs, wave_pairs, bins, active,
-        scan_sums;
+        scan_sums, vkeys, big_rec, big_start;
     Arena a{};
     uint64_t gen = 0;          // bumped whenever a buffer moves: captured graphs are stale
     bool dirty = true;         // bin counts / counters need a k_prep before the next frame
@@ -77,6 +77,7 @@ struct axr_mesh {
     uint32_t ngroups = 0, nmaterials = 0;
     uint64_t nverts = 0, nfaces = 0;
     uint32_t nfaces_sub = 0;   // faces in submission order
+    axr::DevGroup g0{};        // first group (host copy)
     size_t bytes = 0;
 };
 
@@ -98,20 +99,30 @@ int dev_alloc(axr_ctx* ctx, ErrSink& err, void** p, size_t bytes);
 void dev_free(axr_ctx* ctx, void* p, size_t bytes);
 int upload(ErrSink& err, hipStream_t st, void* dst, const void* src, size_t bytes);
 
+// What one launch (a frame or a batch) needs from a frame arena.
+struct FrameSize {
+    uint64_t faces = 0;      // face slots over all views
+    uint64_t tiles = 0;      // bins over all views
+    uint64_t pixels = 0;     // visibility-buffer pixels (0 in binned mode)
+    uint32_t wave_cap = 128; // pairs per setup-wave region (binned mode)
+};
+// High-water marks the kernels reported after an overflow.
+struct ArenaNeeds {
+    uint64_t clip_faces = 0, clip_recs = 0, pairs = 0, big_recs = 0, big_items = 0;
+};
+FrameSize frame_size(const KArgs& ka, uint32_t wave_cap);
+
 int arena_create(axr_ctx* ctx, ErrSink& err, FrameArena& fa);
 void arena_destroy(axr_ctx* ctx, FrameArena& fa);
-// Size the arena for `faces` face slots and `tiles` bins; `need_*` carry
-// high-water marks the kernels reported after an overflow.
-int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uint64_t tiles, uint32_t wave_cap,
-                 uint64_t need_clip_faces = 0, uint64_t need_clip_recs = 0, uint64_t need_pairs = 0);
+uint64_t arena_bytes(const FrameArena& fa);
+int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs, const ArenaNeeds& need = {});
 int arena_read_counters(ErrSink& err, FrameArena& fa);
 // k_prep if the arena is fresh or dirty, then the frame's kernels; ev (may
 // be null) receives the 5 stage events.
 int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev);
 // After arena_read_counters: AXR_OK, or grow the arena to the reported
 // high-water marks, reset it and return AXR_ERR_OVERFLOW.
-int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, uint64_t faces, uint64_t tiles,
-                         uint32_t wave_cap);
+int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs);
 
 // The uniforms drawMesh sets (reference tiled_pipeline.cpp:148-155).
 ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,

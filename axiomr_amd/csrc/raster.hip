@@ -50,6 +50,7 @@ __device__ __forceinline__ uint32_t find_le(const uint32_t* a, uint32_t n, uint3
 }
 
 __device__ __forceinline__ DevGroup group_of(const ViewDesc& V, uint32_t rank) {
+    if (V.ngroups == 1) return V.g0;
     uint32_t lo = 0, hi = V.ngroups;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
@@ -60,6 +61,27 @@ __device__ __forceinline__ DevGroup group_of(const ViewDesc& V, uint32_t rank) {
 
 __device__ __forceinline__ vec3 load_pos(const float* p, uint32_t i) {
     return make3(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]);
+}
+
+// One face as the setup stage sees it: submission rank s within its view,
+// its group, mesh face index, vertex indices and clip-space corners
+// (reference tiled_pipeline.cpp:185-212: gl_Position = mvp * position).
+struct FaceGeo {
+    DevGroup G;
+    uint32_t face, i0, i1, i2;
+    vec4 c0, c1, c2;
+};
+
+__device__ __forceinline__ void face_geo(const ViewDesc& V, uint32_t s, FaceGeo& f) {
+    f.G = group_of(V, s);
+    f.face = f.G.first + (s - f.G.rank_base);
+    const uint32_t* ip = V.idx + 3 * (size_t)f.face;
+    f.i0 = ip[0];
+    f.i1 = ip[1];
+    f.i2 = ip[2];
+    f.c0 = V.mvp * make4(load_pos(V.pos, f.i0), 1.0f);
+    f.c1 = V.mvp * make4(load_pos(V.pos, f.i1), 1.0f);
+    f.c2 = V.mvp * make4(load_pos(V.pos, f.i2), 1.0f);
 }
 
 // Wave-aggregated append: returns this lane's slot for lanes with pred set.
@@ -229,10 +251,11 @@ constexpr unsigned long long EMPTY_KEY = ~0ull;
 // is (re)allocated or after an overflow.
 __global__ void k_prep(Arena ar, uint32_t ntiles) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid < CNT_ERROR) ar.counters[tid] = 0;
+    if (tid < CNT_ERROR || tid == CNT_BIG_ITEMS || tid == CNT_BIG_RECS) ar.counters[tid] = 0;
     for (uint32_t i = tid; i < ntiles; i += gridDim.x * blockDim.x) {
         ar.tile_count[i] = 0;
     }
+    for (uint32_t i = tid; i < ar.cap_vkeys; i += gridDim.x * blockDim.x) ar.vkeys[i] = ~0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -326,6 +349,79 @@ __device__ __forceinline__ void emit_pairs_lane(const Arena& ar, const ViewDesc&
 }
 
 // ---------------------------------------------------------------------------
+// visibility mode: helpers
+
+// Pixels of a large triangle's screen rectangle per work item: one wave,
+// four pixels per lane.
+constexpr uint32_t BIG_CHUNK = 256;
+
+__device__ __forceinline__ unsigned long long* big_counter(const Arena& ar) {
+    return reinterpret_cast<unsigned long long*>(ar.counters + CNT_BIG_ITEMS);
+}
+
+// Mark the bins of rectangle rc as holding visibility keys.
+template <int T>
+__device__ __forceinline__ void flag_bins(const Arena& ar, const ViewDesc& V, const Rect& rc) {
+    const int tx1 = (rc.x1 - 1) / T, ty1 = (rc.y1 - 1) / T;
+    for (int ty = rc.y0 / T; ty <= ty1; ++ty)
+        for (int tx = rc.x0 / T; tx <= tx1; ++tx) ar.tile_count[V.tile_base + (uint32_t)(ty * V.tiles_x + tx)] = 1u;
+}
+
+// One lane rasterizes a whole (small) triangle: every covered pixel of rc
+// takes the minimum of its 64-bit key.
+template <int T>
+__device__ __forceinline__ void vis_raster_rect(const Arena& ar, const ViewDesc& V, const RasterTri& rt, const Rect& rc,
+                                                uint32_t key) {
+    unsigned long long* vk = ar.vkeys + V.pix_base;
+    bool any = false;
+    for (int py = rc.y0; py < rc.y1; ++py)
+        for (int px = rc.x0; px < rc.x1; ++px) {
+            float al, be, ga, z;
+            if (cover(rt, px, py, al, be, ga, z) && z < INFINITY) {
+                atomicMin(&vk[(size_t)py * (size_t)V.W + (size_t)px], depth_key(z, key));
+                any = true;
+            }
+        }
+    if (any) flag_bins<T>(ar, V, rc);
+}
+
+__device__ __forceinline__ uint32_t big_chunks(uint32_t area) { return (area + BIG_CHUNK - 1) / BIG_CHUNK; }
+
+__device__ __forceinline__ void big_store(const Arena& ar, const TriRec& r, uint32_t ri, uint32_t i0, uint32_t nch) {
+    if (ri >= ar.cap_big_recs || i0 + nch > ar.cap_big_items || i0 + nch < i0) {
+        atomicOr(&ar.counters[CNT_ERROR], ERR_BIG);
+        atomicMax(&ar.counters[CNT_NEED_BIG_RECS], ri + 1);
+        atomicMax(&ar.counters[CNT_NEED_BIG_ITEMS], i0 + nch);
+        return;
+    }
+    ar.big_rec[ri] = r;
+    ar.big_start[ri] = i0;
+}
+
+// Queue large triangles (one per flagged lane, whole wave calls): one
+// record each plus ceil(area / BIG_CHUNK) work items.  Records and items
+// come from one 64-bit counter, so item starts grow with the record index.
+__device__ __forceinline__ void big_append_wave(const Arena& ar, bool big, const TriRec& r, uint32_t area) {
+    const unsigned long long mask = __ballot(big);
+    if (mask == 0ull) return;
+    const uint32_t nch = big ? big_chunks(area) : 0u;
+    const uint32_t incl = wave_incl_scan(nch);
+    const uint32_t total = __shfl(incl, 63);
+    unsigned long long old = 0;
+    if (__lane_id() == 0) old = atomicAdd(big_counter(ar), ((unsigned long long)__popcll(mask) << 32) | total);
+    const uint32_t lo = __shfl((uint32_t)old, 0), hi = __shfl((uint32_t)(old >> 32), 0);
+    if (!big) return;
+    big_store(ar, r, hi + (uint32_t)__popcll(mask & __lanemask_lt()), lo + incl - nch, nch);
+}
+
+// Same for one lane on its own (the clipper).
+__device__ __forceinline__ void big_append_lane(const Arena& ar, const TriRec& r, uint32_t area) {
+    const uint32_t nch = big_chunks(area);
+    const unsigned long long old = atomicAdd(big_counter(ar), (1ull << 32) | nch);
+    big_store(ar, r, (uint32_t)(old >> 32), (uint32_t)old, nch);
+}
+
+// ---------------------------------------------------------------------------
 // k_setup: one lane per face (submission order across views).
 
 template <int T, bool BATCH>
@@ -340,25 +436,20 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
         if constexpr (BATCH) vi = find_le(ka.face_bases, ka.nviews, g);
         const ViewDesc& V = view_at<BATCH>(ka, vi);
         const uint32_t s = g - V.face_base;
-        const DevGroup G = group_of(V, s);
-        const uint32_t face = G.first + (s - G.rank_base);
-        const uint32_t* ip = V.idx + 3 * (size_t)face;
-        const uint32_t i0 = ip[0], i1 = ip[1], i2 = ip[2];
-        const vec4 c0 = V.mvp * make4(load_pos(V.pos, i0), 1.0f);
-        const vec4 c1 = V.mvp * make4(load_pos(V.pos, i1), 1.0f);
-        const vec4 c2 = V.mvp * make4(load_pos(V.pos, i2), 1.0f);
-        const int cls = classify(c0, c1, c2);
+        FaceGeo f;
+        face_geo(V, s, f);
+        const int cls = classify(f.c0, f.c1, f.c2);
         want_clip = cls == CLS_CLIP;
         if (cls == CLS_INSIDE) {
-            TriRec r = make_rec(c0, c1, c2, V.W, V.H);
+            TriRec r = make_rec(f.c0, f.c1, f.c2, V.W, V.H);
             if (!is_backface(r)) {
                 RasterTri rt;
                 if (raster_setup(r, rt)) {
                     rc = screen_rect(rt, V.W, V.H);
                     if (rc.x0 < rc.x1 && rc.y0 < rc.y1) {
                         r.c.y = s * 8u;
-                        r.c.z = face;
-                        r.c.w = G.material & REC_MATERIAL_MASK;
+                        r.c.z = f.face;
+                        r.c.w = f.G.material & REC_MATERIAL_MASK;
                         ar.rec[g] = r;
                         binnable = true;
                     }
@@ -372,6 +463,56 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
         else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
     }
     emit_pairs_wave<T>(ar, view_at<BATCH>(ka, vi), binnable, rc, g, g >> 6);
+}
+
+// ---------------------------------------------------------------------------
+// k_vis: visibility mode's setup.  One lane per face: the same transform,
+// classification, cull and setup as k_setup; a triangle whose screen
+// rectangle has at most small_footprint pixels is rasterized by its lane
+// straight into the key buffer, a larger one is queued as work items,
+// faces that cross a plane go to the clipper.
+
+template <int T, bool BATCH>
+__global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (ar.counters[CNT_ERROR]) return;
+    bool want_clip = false, big = false;
+    TriRec r;
+    uint32_t area = 0;
+    if (g < ka.total_faces) {
+        uint32_t vi = 0;
+        if constexpr (BATCH) vi = find_le(ka.face_bases, ka.nviews, g);
+        const ViewDesc& V = view_at<BATCH>(ka, vi);
+        const uint32_t s = g - V.face_base;
+        FaceGeo f;
+        face_geo(V, s, f);
+        const int cls = classify(f.c0, f.c1, f.c2);
+        want_clip = cls == CLS_CLIP;
+        if (cls == CLS_INSIDE) {
+            r = make_rec(f.c0, f.c1, f.c2, V.W, V.H);
+            RasterTri rt;
+            if (!is_backface(r) && raster_setup(r, rt)) {
+                const Rect rc = screen_rect(rt, V.W, V.H);
+                if (rc.x0 < rc.x1 && rc.y0 < rc.y1) {
+                    area = (uint32_t)((rc.x1 - rc.x0) * (rc.y1 - rc.y0));
+                    if (area <= (uint32_t)ka.small_footprint) {
+                        vis_raster_rect<T>(ar, V, rt, rc, s * 8u);
+                    } else {
+                        big = true;
+                        r.c.y = s * 8u;
+                        r.c.z = vi;
+                        r.c.w = f.G.material & REC_MATERIAL_MASK;
+                    }
+                }
+            }
+        }
+    }
+    const uint32_t cslot = wave_append(&ar.counters[CNT_CLIP_FACES], want_clip);
+    if (want_clip) {
+        if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g;
+        else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
+    }
+    big_append_wave(ar, big, r, area);
 }
 
 // ---------------------------------------------------------------------------
@@ -462,7 +603,7 @@ __device__ int clip_triangle(CV* tris, CV* next) {
     return count;
 }
 
-template <int T, bool BATCH>
+template <int T, bool BATCH, bool VIS>
 __global__ __launch_bounds__(64) void k_clip(KArgs ka, Arena ar) {
     if (ar.counters[CNT_ERROR]) return;
     const uint32_t n = imin((int)ar.counters[CNT_CLIP_FACES], (int)ar.cap_clip_faces);
@@ -506,7 +647,13 @@ __global__ __launch_bounds__(64) void k_clip(KArgs ka, Arena ar) {
                 float* dst = reinterpret_cast<float*>(ar.clip_verts + 3 * (size_t)slot + k);
                 for (int i = 0; i < 14; ++i) dst[i] = tris[3 * j + k].f[i];
             }
-            emit_pairs_lane<T>(ar, V, rc, REF_CLIP | slot);
+            if constexpr (VIS) {
+                const uint32_t area = (uint32_t)((rc.x1 - rc.x0) * (rc.y1 - rc.y0));
+                if (area <= (uint32_t)ka.small_footprint) vis_raster_rect<T>(ar, V, rt, rc, r.c.y);
+                else big_append_lane(ar, r, area);
+            } else {
+                emit_pairs_lane<T>(ar, V, rc, REF_CLIP | slot);
+            }
         }
         TriRec m;
         m.a = make_float4(0, 0, 0, 0);
@@ -684,28 +831,48 @@ __device__ __forceinline__ const TriRec& rec_of(const Arena& ar, uint32_t ref) {
     return (ref & REF_CLIP) ? ar.clip_rec[ref & ~REF_CLIP] : ar.rec[ref];
 }
 
+// Vertex stage of the winner's three corners, then its fragment stage
+// (reference tiled_pipeline.cpp:185-212 vertex, 564-575 fragment).  Clipped
+// pieces take their corners from the clip records' vertices.
 template <class S>
-__device__ __forceinline__ vec4 shade_winner(const ViewDesc& V, const Arena& ar, const TriRec& r, uint32_t slot,
-                                            bool clipped, uint32_t rank, vec3 bar) {
+__device__ __forceinline__ vec4 shade_at(const ViewDesc& V, const Arena& ar, uint32_t material, uint32_t prim,
+                                         bool clipped, uint32_t slot, uint32_t i0, uint32_t i1, uint32_t i2,
+                                         vec3 bar) {
     ShadeCtx c;
     c.u = &V.u;
-    c.m = &V.materials[r.c.w & REC_MATERIAL_MASK];
+    c.m = &V.materials[material & REC_MATERIAL_MASK];
     c.face_colors = V.face_colors;
+    c.prim = prim;
     typename S::Var var[3];
-    if (clipped) {
-        const DevGroup G = group_of(V, rank);
-        c.prim = G.first + (rank - G.rank_base);
-        if constexpr (sizeof(typename S::Var) > 1) {
+    if constexpr (sizeof(typename S::Var) > 1) {
+        if (clipped) {
             for (int k = 0; k < 3; ++k) var[k] = S::vs(ar.clip_verts[3 * (size_t)slot + k], V.u);
-        }
-    } else {
-        c.prim = r.c.z;
-        if constexpr (sizeof(typename S::Var) > 1) {
-            const uint32_t* ip = V.idx + 3 * (size_t)r.c.z;
-            for (int k = 0; k < 3; ++k) var[k] = S::vs(V.verts[ip[k]], V.u);
+        } else {
+            var[0] = S::vs(V.verts[i0], V.u);
+            var[1] = S::vs(V.verts[i1], V.u);
+            var[2] = S::vs(V.verts[i2], V.u);
         }
     }
     return S::fs(bar, var, c);
+}
+
+template <class S>
+__device__ __forceinline__ vec4 shade_winner(const ViewDesc& V, const Arena& ar, const TriRec& r, uint32_t slot,
+                                            bool clipped, uint32_t rank, vec3 bar) {
+    uint32_t prim, i0 = 0, i1 = 0, i2 = 0;
+    if (clipped) {
+        const DevGroup G = group_of(V, rank);
+        prim = G.first + (rank - G.rank_base);
+    } else {
+        prim = r.c.z;
+        if constexpr (sizeof(typename S::Var) > 1) {
+            const uint32_t* ip = V.idx + 3 * (size_t)prim;
+            i0 = ip[0];
+            i1 = ip[1];
+            i2 = ip[2];
+        }
+    }
+    return shade_at<S>(V, ar, r.c.w, prim, clipped, slot, i0, i1, i2, bar);
 }
 
 template <class S>
@@ -839,6 +1006,177 @@ __global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
     raster_bin<T, NT, BATCH>(ka, ar, ar.active[a], keys, bigq, nbig);
 }
 
+// Largest i in [0, n) with a[i] <= x (a ascending, a[0] <= x), searched by
+// the whole wave: 64 probes per round, so about log64(n) dependent loads.
+// Every lane of the wave must call it with the same arguments.
+__device__ __forceinline__ uint32_t wave_find_le(const uint32_t* a, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    const uint32_t lane = __lane_id();
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t p = lo + lane * step;
+        const bool le = p < hi && a[p] <= x;
+        const unsigned long long m = __ballot(le);      // a prefix of the lanes (a is ascending)
+        const uint32_t last = 63u - (uint32_t)__clzll((long long)m);
+        lo = lo + last * step;
+        hi = lo + step < hi ? lo + step : hi;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// k_big: visibility mode, the work items of large triangles.  One wave per
+// item: BIG_CHUNK consecutive pixels (row-major) of the triangle's screen
+// rectangle, four per lane.
+
+template <int T, bool BATCH>
+__global__ __launch_bounds__(256) void k_big(KArgs ka, Arena ar) {
+    if (ar.counters[CNT_ERROR]) return;
+    const unsigned long long c = *big_counter(ar);
+    const uint32_t nitems = imin((int)(uint32_t)c, (int)ar.cap_big_items);
+    const uint32_t nrecs = imin((int)(uint32_t)(c >> 32), (int)ar.cap_big_recs);
+    const uint32_t lane = __lane_id();
+    const uint32_t nw = gridDim.x * 4u;
+    for (uint32_t it = (blockIdx.x * 256u + threadIdx.x) >> 6; it < nitems; it += nw) {
+        const uint32_t ri = wave_find_le(ar.big_start, nrecs, it);
+        const TriRec r = ar.big_rec[ri];
+        uint32_t vi = 0;
+        if constexpr (BATCH) vi = r.c.z;
+        const ViewDesc& V = view_at<BATCH>(ka, vi);
+        RasterTri rt;
+        raster_setup(r, rt);
+        const Rect rc = screen_rect(rt, V.W, V.H);
+        const uint32_t w = (uint32_t)(rc.x1 - rc.x0);
+        const uint32_t np = w * (uint32_t)(rc.y1 - rc.y0);
+        const uint32_t p0 = (it - ar.big_start[ri]) * BIG_CHUNK;
+        const uint32_t p1 = p0 + BIG_CHUNK < np ? p0 + BIG_CHUNK : np;
+        unsigned long long* vk = ar.vkeys + V.pix_base;
+        const uint32_t key = r.c.y;
+        for (uint32_t p = p0 + lane; p < p1; p += 64u) {
+            const int px = rc.x0 + (int)(p % w), py = rc.y0 + (int)(p / w);
+            float al, be, ga, z;
+            if (cover(rt, px, py, al, be, ga, z) && z < INFINITY) {
+                atomicMin(&vk[(size_t)py * (size_t)V.W + (size_t)px], depth_key(z, key));
+                ar.tile_count[V.tile_base + (uint32_t)((py / T) * V.tiles_x + px / T)] = 1u;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_resolve: visibility mode's last stage.  One workgroup per bin; bins
+// without keys exit at once.  Every pixel with a key: recompute the winner's
+// setup from its face (or read its clipped piece), depth-test against the
+// framebuffer (strict less, reference mergeTileResults), shade, store, and
+// leave the key empty for the next frame.
+
+template <class S>
+__device__ __forceinline__ void resolve_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
+                                                  int py, size_t pix) {
+    const float dfb = V.depth[pix];
+    const uint32_t key = (uint32_t)k;
+    const uint32_t s = key >> 3, j = key & 7u;
+    const DevGroup G = group_of(V, s);
+    const uint32_t face = G.first + (s - G.rank_base);
+    const uint32_t* ip = V.idx + 3 * (size_t)face;
+    const uint32_t i0 = ip[0], i1 = ip[1], i2 = ip[2];
+    // the shaders' vertex stage reads the whole vertex; its first three
+    // floats are the position (the packed copy setup read holds the same bits)
+    constexpr bool kVerts = sizeof(typename S::Var) > 1;
+    Vertex56 v0, v1, v2;
+    vec3 p0, p1, p2;
+    if constexpr (kVerts) {
+        v0 = V.verts[i0];
+        v1 = V.verts[i1];
+        v2 = V.verts[i2];
+        p0 = make3(v0.px, v0.py, v0.pz);
+        p1 = make3(v1.px, v1.py, v1.pz);
+        p2 = make3(v2.px, v2.py, v2.pz);
+    } else {
+        p0 = load_pos(V.pos, i0);
+        p1 = load_pos(V.pos, i1);
+        p2 = load_pos(V.pos, i2);
+    }
+    const vec4 c0 = V.mvp * make4(p0, 1.0f), c1 = V.mvp * make4(p1, 1.0f), c2 = V.mvp * make4(p2, 1.0f);
+    TriRec r;
+    bool clipped = false;
+    uint32_t slot = 0;
+    if (classify(c0, c1, c2) == CLS_CLIP) {
+        clipped = true;
+        slot = ar.rec[V.face_base + s].c.z + j;   // redirect written by k_clip
+        r = ar.clip_rec[slot];
+    } else {
+        r = make_rec(c0, c1, c2, V.W, V.H);
+    }
+    RasterTri rt;
+    raster_setup(r, rt);
+    float al, be, ga, z;
+    cover(rt, px, py, al, be, ga, z);
+    if (!(z < dfb)) return;   // reference mergeTileResults: strict less
+    ShadeCtx c;
+    c.u = &V.u;
+    c.m = &V.materials[G.material & REC_MATERIAL_MASK];
+    c.face_colors = V.face_colors;
+    c.prim = face;
+    typename S::Var var[3];
+    if constexpr (kVerts) {
+        if (clipped) {
+            for (int q = 0; q < 3; ++q) var[q] = S::vs(ar.clip_verts[3 * (size_t)slot + q], V.u);
+        } else {
+            var[0] = S::vs(v0, V.u);
+            var[1] = S::vs(v1, V.u);
+            var[2] = S::vs(v2, V.u);
+        }
+    }
+    const vec4 col = S::fs(make3(al, be, ga), var, c);
+    V.depth[pix] = z;
+    V.color[pix] = pack_bgra(col);
+    if (V.ids) V.ids[pix] = key;
+}
+
+template <int T, int NT, class S>
+__device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena& ar, int ox, int oy) {
+    unsigned long long* vk = ar.vkeys + V.pix_base;
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        const int px = ox + (p % T), py = oy + (p / T);
+        if (px >= V.W || py >= V.H) continue;
+        const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
+        const unsigned long long k = vk[pix];
+        if (k == EMPTY_KEY) continue;
+        vk[pix] = EMPTY_KEY;
+        resolve_vis_pixel<S>(V, ar, k, px, py, pix);
+    }
+}
+
+template <int T, int NT, bool BATCH>
+__global__ __launch_bounds__(NT) void k_resolve(KArgs ka, Arena ar) {
+    if (ar.counters[CNT_ERROR]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the per-frame counters of this mode were last needed by k_big
+        ar.counters[CNT_CLIP_FACES] = 0;
+        ar.counters[CNT_CLIP_RECS] = 0;
+        *big_counter(ar) = 0ull;
+    }
+    for (uint32_t t = blockIdx.x; t < ka.total_tiles; t += gridDim.x) {
+        if (ar.tile_count[t] == 0u) continue;   // the same word for every thread
+        __syncthreads();
+        if (threadIdx.x == 0) ar.tile_count[t] = 0u;
+        uint32_t vi = 0;
+        if constexpr (BATCH) vi = ka.tile_views[t];
+        const ViewDesc& V = view_at<BATCH>(ka, vi);
+        const uint32_t local = t - V.tile_base;
+        const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
+        switch (V.u.kind) {
+        case SHADER_FLAT: resolve_vis_tile<T, NT, FlatShader>(V, ar, ox, oy); break;
+        case SHADER_PHONG: resolve_vis_tile<T, NT, PhongShader>(V, ar, ox, oy); break;
+        case SHADER_PBR: resolve_vis_tile<T, NT, PBRShader>(V, ar, ox, oy); break;
+        case SHADER_GOURAUD: resolve_vis_tile<T, NT, GouraudShader>(V, ar, ox, oy); break;
+        case SHADER_PHONG_POINT: resolve_vis_tile<T, NT, PhongPointShader>(V, ar, ox, oy); break;
+        default: resolve_vis_tile<T, NT, FlatColorShader>(V, ar, ox, oy); break;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // framebuffer clear (reference framebuffer.cpp:26-42)
 
@@ -874,13 +1212,33 @@ struct LaunchShape {
 // ev (optional): 5 events recorded after setup, clip, scan, scatter and
 // raster, on the stream the kernels run on.  Expects a clean arena (bin
 // counts and per-frame counters zero) and leaves it clean.
+constexpr int RESOLVE_MAX_BLOCKS = 32768;
+
+// Visibility mode: k_vis, k_clip, k_big, k_resolve.  Events: after k_vis,
+// k_clip, k_big, (nothing), k_resolve.
+template <int T, int NT, bool BATCH>
+static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+    const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
+    if (setup_blocks) k_vis<T, BATCH><<<setup_blocks, 256, 0, st>>>(ka, ar);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    k_clip<T, BATCH, true><<<256, 64, 0, st>>>(ka, ar);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    k_big<T, BATCH><<<1024, 256, 0, st>>>(ka, ar);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    k_resolve<T, NT, BATCH><<<imax(1, imin((int)ka.total_tiles, RESOLVE_MAX_BLOCKS)), NT, 0, st>>>(ka, ar);
+    if (ev) (void)hipEventRecord(ev[4], st);
+    return hipGetLastError();
+}
+
 template <int T, int NT, bool BATCH>
 static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+    if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, NT, BATCH>(ka, ar, st, ev);
     const uint32_t ntiles = ka.total_tiles;
     const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
     if (setup_blocks) k_setup<T, BATCH><<<setup_blocks, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[0], st);
-    k_clip<T, BATCH><<<256, 64, 0, st>>>(ka, ar);
+    k_clip<T, BATCH, false><<<256, 64, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[1], st);
     if (ntiles <= SMALL_SCAN_TILES) {
         k_scan_single<<<1, SCAN_THREADS, 0, st>>>(ar, ntiles);

@@ -46,7 +46,7 @@ public:
         uint32_t* h_tile_views = nullptr;
         hipEvent_t done = nullptr;
         KArgs ka{};
-        uint64_t faces = 0, tiles = 0;
+        FrameSize fs;
         int attempts = 0;
     };
 
@@ -55,6 +55,7 @@ public:
         wave_cap_ = (uint32_t)ctx->cfg.bin_capacity;
         raster_threads_ = ctx->cfg.raster_threads;
         small_footprint_ = ctx->cfg.small_footprint;
+        mode_ = ctx->cfg.raster_mode;
         lanes_.resize(cfg.lanes);
     }
 
@@ -78,7 +79,12 @@ public:
             int r = arena_create(ctx_, err_, l.fa);
             if (r) return r;
             // sized for a full batch up front so that serving never grows it
-            if ((r = arena_ensure(ctx_, err_, l.fa, cfg_.max_batch_faces, cfg_.max_batch_tiles, wave_cap_))) return r;
+            FrameSize fs;
+            fs.faces = cfg_.max_batch_faces;
+            fs.tiles = cfg_.max_batch_tiles;
+            fs.pixels = mode_ == RASTER_VISIBILITY ? cfg_.max_batch_tiles * (uint64_t)(tile_ * tile_) : 0;
+            fs.wave_cap = wave_cap_;
+            if ((r = arena_ensure(ctx_, err_, l.fa, fs))) return r;
             if ((r = dev_alloc(ctx_, err_, (void**)&l.d_views, views_bytes())) ||
                 (r = dev_alloc(ctx_, err_, (void**)&l.d_bases, bases_bytes())) ||
                 (r = dev_alloc(ctx_, err_, (void**)&l.d_tile_views, tiles_bytes())))
@@ -98,13 +104,7 @@ public:
 
     uint64_t fixed_bytes() const override {
         uint64_t b = 0;
-        for (const Lane& l : lanes_) {
-            const FrameArena& f = l.fa;
-            for (const DevBuf* d : {&f.counters, &f.rec, &f.clip_faces, &f.clip_rec, &f.clip_verts, &f.tile_count,
-                                    &f.tile_off, &f.pairs, &f.wave_pairs, &f.bins, &f.active, &f.scan_sums})
-                b += d->bytes;
-            b += views_bytes() + bases_bytes() + tiles_bytes();
-        }
+        for (const Lane& l : lanes_) b += arena_bytes(l.fa) + views_bytes() + bases_bytes() + tiles_bytes();
         return b;
     }
 
@@ -125,6 +125,7 @@ public:
                                    tile_);
             v.face_base = (uint32_t)faces;
             v.tile_base = (uint32_t)tiles;
+            v.pix_base = (uint32_t)pixels;
             L.h_views[i] = v;
             L.h_bases[i] = (uint32_t)faces;
             L.h_bases[stride + i] = (uint32_t)tiles;
@@ -147,6 +148,8 @@ public:
         ka.tile_size = tile_;
         ka.raster_threads = raster_threads_;
         ka.small_footprint = small_footprint_;
+        ka.mode = mode_;
+        ka.total_pixels = (uint32_t)pixels;
         ka.one = L.h_views[0];
         hipStream_t st = L.fa.stream;
         hipError_t e;
@@ -171,10 +174,9 @@ public:
                                        __builtin_huge_valf(), st);
         }
         if (e != hipSuccess) return hip_fail(err_, e, "server batch upload");
-        L.faces = faces;
-        L.tiles = tiles;
+        L.fs = frame_size(ka, wave_cap_);
         L.attempts = 0;
-        int r = arena_ensure(ctx_, err_, L.fa, faces, tiles, wave_cap_);
+        int r = arena_ensure(ctx_, err_, L.fa, L.fs);
         if (r) return r;
         return enqueue(L);
     }
@@ -188,7 +190,7 @@ public:
         // arena overflow: the raster stage did not run, so the cleared
         // framebuffers are untouched; grow to the reported needs and redraw
         if (++L.attempts > 8) return fail(err_, AXR_ERR_OVERFLOW, "server batch kept overflowing its arena");
-        int r = arena_check_overflow(ctx_, err_, L.fa, L.faces, L.tiles, wave_cap_);
+        int r = arena_check_overflow(ctx_, err_, L.fa, L.fs);
         if (r != AXR_ERR_OVERFLOW) return r < 0 ? r : AXR_ERR_OVERFLOW;
         ++overflow_retries_;
         r = enqueue(L);
@@ -216,7 +218,7 @@ private:
     axr_server_config cfg_;
     ErrSink err_;
     std::vector<Lane> lanes_;
-    int tile_ = 16, raster_threads_ = 64, small_footprint_ = 12;
+    int tile_ = 16, raster_threads_ = 64, small_footprint_ = 12, mode_ = RASTER_BINNED;
     uint32_t wave_cap_ = 128;
     uint64_t overflow_retries_ = 0;
 };
@@ -375,6 +377,12 @@ int axr_server_release(axr_server* s, uint64_t ticket) {
     const int r = s->core->release(ticket);
     if (r == AXR_ERR_INVALID) return server_fail(s, r, "unknown or released ticket");
     if (r) return server_fail(s, r, "request still queued or running");
+    return AXR_OK;
+}
+
+int axr_server_pause(axr_server* s, int pause) {
+    if (!s) return AXR_ERR_INVALID;
+    s->core->pause(pause != 0);
     return AXR_OK;
 }
 

@@ -249,6 +249,16 @@ public:
         return s;
     }
 
+    // While paused, requests queue but none is admitted; resuming admits
+    // the queue in order (so a burst submitted while paused is batched).
+    void pause(bool on) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            paused_ = on;
+        }
+        cv_.notify_all();
+    }
+
     std::string last_error() {
         std::lock_guard<std::mutex> lk(mu_);
         return err_;
@@ -383,6 +393,7 @@ private:
     }
 
     bool admit() {
+        if (paused_) return false;
         bool changed = false;
         std::vector<JobT*> batch;
         for (int l = 0; l < (int)lanes_.size() && !queue_.empty(); ++l) {
@@ -455,6 +466,7 @@ private:
     std::condition_variable cv_, done_cv_;
     std::thread thread_;
     bool running_ = false;
+    bool paused_ = false;
     std::vector<JobT> jobs_;
     std::vector<uint32_t> free_slots_;
     std::deque<uint32_t> queue_;

@@ -75,7 +75,7 @@ class Context:
         cfg = N.AxrConfig()
         self.check(self._lib.axr_ctx_get_config(self.handle, C.byref(cfg)))
         return {"tile_size": cfg.tile_size, "raster_threads": cfg.raster_threads, "bin_capacity": cfg.bin_capacity,
-                "small_footprint": cfg.small_footprint}
+                "small_footprint": cfg.small_footprint, "raster_mode": cfg.raster_mode}
 
     def set_config(self, **kw) -> None:
         cfg = N.AxrConfig()

@@ -89,6 +89,10 @@ class Server:
         self.check(self._lib.axr_server_release(self.handle, ticket))
         self._sizes.pop(ticket, None)
 
+    def pause(self, on: bool = True) -> None:
+        """Queue without admitting (on) / admit the queue in order (off)."""
+        self.check(self._lib.axr_server_pause(self.handle, 1 if on else 0))
+
     def stats(self) -> dict:
         s = N.AxrServerStats()
         self.check(self._lib.axr_server_get_stats(self.handle, C.byref(s)))

@@ -1,10 +1,12 @@
 """Tune the binning pipeline for the benchmark workloads on this GPU and
 store the result in axiomr_amd/tuning/mi355x.json:
 
-    python -m axiomr_amd.tune_workloads [plain] [batch]
+    python -m axiomr_amd.tune_workloads [plain] [batch] [serve]
 
 plain: one 1920x1080 frame of the 1,003,520-triangle icosphere;
-batch: the 256-view / 32-mesh batch in one graph launch.
+batch: the 256-view / 32-mesh batch in one graph launch;
+serve: a burst of 400 requests of the served workload (64 meshes, mixed
+resolutions) through a 3-lane render server.
 """
 from __future__ import annotations
 
@@ -53,11 +55,32 @@ def tune_batch(ctx):
     return res
 
 
+def tune_serve(ctx):
+    import numpy as np
+    from .serve import Server
+    wl = S.serve_workload()
+    gs = RS.GpuServeScene(ctx, wl)
+    reqs = [gs.request(v) for v in wl.requests(400, seed=11)]
+    zeros = np.zeros(len(reqs))
+    holder = {}
+
+    def make_frame():
+        if "srv" in holder:
+            holder["srv"].close()
+        holder["srv"] = srv = Server(ctx, lanes=3, max_batch_views=64)
+        return lambda: srv.replay(reqs, zeros)
+
+    res = AT.tune(ctx, make_frame, log, reps=3, inner=1)
+    holder["srv"].close()
+    gs.close()
+    return res
+
+
 def main(argv):
-    which = argv or ["plain", "batch"]
+    which = argv or ["plain", "batch", "serve"]
     ctx = Context(0)
     for w in which:
-        res = {"plain": tune_plain, "batch": tune_batch}[w](ctx)
+        res = {"plain": tune_plain, "batch": tune_batch, "serve": tune_serve}[w](ctx)
         log(f"[autotune] {w}: best {res['best']} {res['seconds'] * 1e3:.3f} ms")
         AT.store(w, res)
         print(f"{w} {res['best']} {res['seconds'] * 1e3:.4f} ms", flush=True)

@@ -57,16 +57,28 @@ def setup_bytes(nverts, nfaces):
     return 12 * nfaces + 12 * nverts
 
 
+RASTER_MODE = None   # --raster-mode: overrides the tuned or default mode
+
+
 def apply_config(ctx, workload, tile, use_tuned):
     """The tuned tunables for `workload` when cached (and asked for), else
-    the defaults with --tile."""
+    the defaults with --tile; then --raster-mode if given."""
     from axiomr_amd import autotune as AT
     cached = AT.load(workload) if use_tuned else None
     if cached:
         ctx.set_config(**cached["config"])
     elif tile:
         ctx.set_config(tile_size=tile)
+    if RASTER_MODE is not None:
+        ctx.set_config(raster_mode=RASTER_MODE)
     return ctx.config()
+
+
+def stage_names(cfg):
+    """Names of the five event-timed stages in the context's raster mode."""
+    if cfg.get("raster_mode", 0) == 1:
+        return ("vis", "clip", "big", "none", "resolve")
+    return ("setup", "clip", "scan", "scatter", "raster")
 
 
 def run_plain(scene, steps, warmup, tile, use_tuned):
@@ -112,7 +124,7 @@ def run_plain(scene, steps, warmup, tile, use_tuned):
     ctx.sync()
     st = ctx.stats()
     ctx.set_profiling(False)
-    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(st.STAGES)}
+    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(stage_names(cfg))}
     frame_ms = st.frame_ms_total / max(1, st.frames)
     in_use, arena = ctx.memory()
     fb.close()
@@ -165,7 +177,7 @@ def run_batch(wl, reps, tile, use_tuned):
     ctx.sync()
     st = ctx.stats()
     ctx.set_profiling(False)
-    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(st.STAGES)}
+    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(stage_names(cfg))}
     gb.close()
     ctx.close()
     return dict(views=n, plain_views_per_s=n / plain_s, batch_views_per_s=n / batch_s, speedup=plain_s / batch_s,
@@ -338,6 +350,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="stream length per arrival rate")
     ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--raster-mode", choices=("binned", "visibility"), default=None,
+                    help="override the tuned raster mode")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain-copy kernel used to scale the --pmc byte counters")
     args = ap.parse_args()
@@ -346,6 +360,9 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU design is described but not run", file=sys.stderr)
         sys.exit(2)
+    global RASTER_MODE
+    if args.raster_mode:
+        RASTER_MODE = {"binned": 0, "visibility": 1}[args.raster_mode]
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -366,11 +383,11 @@ def main():
 
     stages = r["stages_ms"]
     dominant = max(stages, key=stages.get)
-    if dominant == "setup":
-        kbytes, kname = setup_bytes(nverts, nfaces), "k_setup"
+    if dominant in ("setup", "vis"):
+        kbytes, kname = setup_bytes(nverts, nfaces), "k_" + dominant
     else:
-        kbytes, kname = raster_bytes(r["covered"], ntex), "k_raster"
-        dominant = "raster"
+        dominant = "resolve" if "resolve" in stages else "raster"
+        kbytes, kname = raster_bytes(r["covered"], ntex), "k_" + dominant
     kms = stages[dominant]
     achieved = kbytes / (kms * 1e-3) / 1e9
     frame_bytes = algorithmic_bytes(nverts, nfaces, W, H, r["covered"], ntex)

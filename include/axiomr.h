@@ -111,13 +111,24 @@ typedef struct axr_config {
                                     region, 64..1024 (default 128); more go to a shared pool */
     int32_t small_footprint;     /* triangles covering at most this many pixels of a bin are
                                     walked by one lane, 1..1024 (default 12) */
-    int32_t reserved[4];
+    int32_t raster_mode;         /* AXR_RASTER_BINNED (default) or AXR_RASTER_VISIBILITY */
+    int32_t reserved[3];
 } axr_config;
 
+/* Raster modes; both give the same pixels.  BINNED: setup, per-bin lists
+ * (prefix sum + scatter), one workgroup per bin rasterizes into LDS and
+ * resolves.  VISIBILITY: setup rasterizes small triangles straight into a
+ * per-pixel 64-bit (depth, submission) key buffer with atomics, large ones
+ * as work items; a per-bin pass shades the winners.  tile_size is the
+ * resolve bin there; bin_capacity is unused. */
+enum { AXR_RASTER_BINNED = 0, AXR_RASTER_VISIBILITY = 1 };
+
 /* Stage timing collected with HIP events (recorded on the context's stream,
- * between the kernels) when profiling is on.  Stages: 0 setup (counter reset,
- * transform, classification, cull, triangle setup, bin counts), 1 clip,
- * 2 prefix sum of the bin counts, 3 bin scatter, 4 per-bin raster + resolve. */
+ * between the kernels) when profiling is on.  Binned mode: 0 setup
+ * (transform, classification, cull, triangle setup, bin counts), 1 clip,
+ * 2 prefix sum of the bin counts, 3 bin scatter, 4 per-bin raster +
+ * resolve.  Visibility mode: 0 setup + small-triangle raster, 1 clip,
+ * 2 large-triangle work items, 3 (empty), 4 resolve. */
 typedef struct axr_kernel_stats {
     uint64_t frames;
     double frame_ms_total;
@@ -240,6 +251,9 @@ int axr_server_wait(axr_server* server, uint64_t ticket, double timeout_s);
 int axr_server_read(axr_server* server, uint64_t ticket, uint8_t* bgra, float* depth, uint32_t* ids);
 /* Returns the request's framebuffer to the pool; the ticket becomes invalid. */
 int axr_server_release(axr_server* server, uint64_t ticket);
+/* pause != 0: requests keep queueing but none is admitted; 0: admit the
+ * queue in order again (a burst submitted while paused is batched). */
+int axr_server_pause(axr_server* server, int pause);
 int axr_server_get_stats(axr_server* server, axr_server_stats* out);
 /* Open-loop driver: request i is submitted at (call time + arrival_s[i]),
  * non-decreasing; results are released as they complete and latency_s[i]

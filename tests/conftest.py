@@ -31,10 +31,15 @@ def built_libraries():
     _ensure_built()
 
 
-@pytest.fixture(scope="session")
-def ctx():
-    """One GPU context for the whole session (GPU tests only)."""
+RASTER_MODES = {"binned": 0, "visibility": 1}
+
+
+@pytest.fixture(scope="session", params=list(RASTER_MODES))
+def ctx(request):
+    """One GPU context per raster mode (GPU tests only): every GPU test runs
+    in both modes, which must give the same pixels."""
     import axiomr_amd as axr
     c = axr.Context(0)
+    c.set_config(raster_mode=RASTER_MODES[request.param])
     yield c
     c.close()

@@ -248,6 +248,31 @@ static void test_release_rules() {
     std::puts("release_rules ok");
 }
 
+static void test_pause_batches_a_burst() {
+    FakeDevice dev(2, 0, 0.0);
+    CoreConfig cc;
+    cc.max_views = 4;
+    ServerCore<Req> core(cc, &dev);
+    core.pause(true);
+    std::vector<uint64_t> t(10);
+    for (int i = 0; i < 10; ++i) {
+        Req r;
+        r.id = i;
+        CHECK(core.submit(r, MB, 1, 1, now_s(), false, nullptr, &t[i]) == S_OK);
+    }
+    CHECK(!core.step() && core.queued() == 10 && dev.batches.empty());   // nothing admitted while paused
+    core.pause(false);
+    core.step();
+    CHECK(dev.batches.size() == 2 && dev.batches[0].size() == 4 && dev.batches[1].size() == 4);
+    for (int k = 0; k < 100 && core.queued(); ++k) core.step();
+    while (core.state(t[9]) != JOB_DONE) core.step();
+    CHECK(dev.batches.size() == 3 && dev.batches[2].size() == 2);
+    CoreStats s = core.stats();
+    CHECK(s.max_lanes_busy == 2 && s.launches_overlapped == 1);
+    core.release_pool();
+    std::puts("pause ok");
+}
+
 static void test_replay_open_loop() {
     FakeDevice dev(3, 0, 0.0003);
     ServerCore<Req> core(CoreConfig{}, &dev);
@@ -306,6 +331,7 @@ int main() {
     test_eviction_of_idle_blocks();
     test_failures();
     test_release_rules();
+    test_pause_batches_a_burst();
     test_replay_open_loop();
     test_concurrent_clients();
     std::puts("ALL OK");

@@ -42,7 +42,9 @@ def test_mix_equals_alone(ctx, served):
     wl, gs = served
     views = _requests(wl, 48, 1)
     with Server(ctx, lanes=3, max_batch_views=8, with_ids=True) as srv:
+        srv.pause(True)           # the whole stream queues, then is admitted at once
         tickets = [srv.submit(gs.meshes[v.mesh], v.width, v.height, gs.params(v)) for v in views]
+        srv.pause(False)
         for t in tickets:
             srv.wait(t, 60.0)
             assert srv.poll(t) == DONE
@@ -51,8 +53,9 @@ def test_mix_equals_alone(ctx, served):
         for t in tickets:
             srv.release(t)
     assert st["completed"] == 48 and st["failed"] == 0
-    assert st["batches"] < 48, st               # requests shared batches
-    assert st["max_views_in_batch"] <= 8
+    assert st["batches"] <= 48 // 8 + 3, st     # requests shared batches
+    assert st["max_views_in_batch"] == 8
+    assert st["max_lanes_busy"] == 3 and st["launches_overlapped"] >= 2, st
     covered = 0
     for i, v in enumerate(views):
         alone = gs.draw_plain(v, with_ids=True)
@@ -78,7 +81,9 @@ def test_full_resolution_mix(ctx, served):
     wl, gs = served
     views = _requests(wl, 24, 3, S.SERVE_RESOLUTIONS)
     with Server(ctx, lanes=4, max_batch_views=6) as srv:
+        srv.pause(True)
         tickets = [srv.submit(gs.meshes[v.mesh], v.width, v.height, gs.params(v)) for v in views]
+        srv.pause(False)
         for t in tickets:
             srv.wait(t, 60.0)
         results = [srv.read(t) for t in tickets]
