@@ -472,47 +472,84 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
 // straight into the key buffer, a larger one is queued as work items,
 // faces that cross a plane go to the clipper.
 
+// Faces per lane (their loads issued together: indices, then positions); a
+// 256-thread block covers 256 * VIS_FACES_PER_LANE consecutive faces.  Two
+// per lane measured slower than one on MI355X (fewer waves for small
+// draws, no gain at 1M faces).
+constexpr uint32_t VIS_FACES_PER_LANE = 1;
+
 template <int T, bool BATCH>
 __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
-    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    constexpr uint32_t F = VIS_FACES_PER_LANE;
     if (ar.counters[CNT_ERROR]) return;
-    bool want_clip = false, big = false;
-    TriRec r;
-    uint32_t area = 0;
-    if (g < ka.total_faces) {
-        uint32_t vi = 0;
-        if constexpr (BATCH) vi = find_le(ka.face_bases, ka.nviews, g);
-        const ViewDesc& V = view_at<BATCH>(ka, vi);
-        const uint32_t s = g - V.face_base;
-        FaceGeo f;
-        face_geo(V, s, f);
-        const int cls = classify(f.c0, f.c1, f.c2);
-        want_clip = cls == CLS_CLIP;
-        if (cls == CLS_INSIDE) {
-            r = make_rec(f.c0, f.c1, f.c2, V.W, V.H);
-            RasterTri rt;
-            if (!is_backface(r) && raster_setup(r, rt)) {
-                const Rect rc = screen_rect(rt, V.W, V.H);
-                if (rc.x0 < rc.x1 && rc.y0 < rc.y1) {
-                    area = (uint32_t)((rc.x1 - rc.x0) * (rc.y1 - rc.y0));
-                    if (area <= (uint32_t)ka.small_footprint) {
-                        vis_raster_rect<T>(ar, V, rt, rc, s * 8u);
-                    } else {
-                        big = true;
-                        r.c.y = s * 8u;
-                        r.c.z = vi;
-                        r.c.w = f.G.material & REC_MATERIAL_MASK;
+    uint32_t g[F], vi[F], s[F], face[F], i0[F], i1[F], i2[F];
+    DevGroup G[F];
+    bool act[F];
+#pragma unroll
+    for (uint32_t k = 0; k < F; ++k) {
+        g[k] = blockIdx.x * (256u * F) + k * 256u + threadIdx.x;
+        act[k] = g[k] < ka.total_faces;
+        vi[k] = 0;
+        if constexpr (BATCH) { if (act[k]) vi[k] = find_le(ka.face_bases, ka.nviews, g[k]); }
+        const ViewDesc& V = view_at<BATCH>(ka, vi[k]);
+        s[k] = g[k] - V.face_base;
+        if (act[k]) {
+            G[k] = group_of(V, s[k]);
+            face[k] = G[k].first + (s[k] - G[k].rank_base);
+            const uint32_t* ip = V.idx + 3 * (size_t)face[k];
+            i0[k] = ip[0];
+            i1[k] = ip[1];
+            i2[k] = ip[2];
+        }
+    }
+    vec3 p0[F], p1[F], p2[F];
+#pragma unroll
+    for (uint32_t k = 0; k < F; ++k) {
+        if (act[k]) {
+            const ViewDesc& V = view_at<BATCH>(ka, vi[k]);
+            p0[k] = load_pos(V.pos, i0[k]);
+            p1[k] = load_pos(V.pos, i1[k]);
+            p2[k] = load_pos(V.pos, i2[k]);
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < F; ++k) {
+        bool want_clip = false, big = false;
+        TriRec r;
+        uint32_t area = 0;
+        if (act[k]) {
+            const ViewDesc& V = view_at<BATCH>(ka, vi[k]);
+            const vec4 c0 = V.mvp * make4(p0[k], 1.0f);
+            const vec4 c1 = V.mvp * make4(p1[k], 1.0f);
+            const vec4 c2 = V.mvp * make4(p2[k], 1.0f);
+            const int cls = classify(c0, c1, c2);
+            want_clip = cls == CLS_CLIP;
+            if (cls == CLS_INSIDE) {
+                r = make_rec(c0, c1, c2, V.W, V.H);
+                RasterTri rt;
+                if (!is_backface(r) && raster_setup(r, rt)) {
+                    const Rect rc = screen_rect(rt, V.W, V.H);
+                    if (rc.x0 < rc.x1 && rc.y0 < rc.y1) {
+                        area = (uint32_t)((rc.x1 - rc.x0) * (rc.y1 - rc.y0));
+                        if (area <= (uint32_t)ka.small_footprint) {
+                            vis_raster_rect<T>(ar, V, rt, rc, s[k] * 8u);
+                        } else {
+                            big = true;
+                            r.c.y = s[k] * 8u;
+                            r.c.z = vi[k];
+                            r.c.w = G[k].material & REC_MATERIAL_MASK;
+                        }
                     }
                 }
             }
         }
+        const uint32_t cslot = wave_append(&ar.counters[CNT_CLIP_FACES], want_clip);
+        if (want_clip) {
+            if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g[k];
+            else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
+        }
+        big_append_wave(ar, big, r, area);
     }
-    const uint32_t cslot = wave_append(&ar.counters[CNT_CLIP_FACES], want_clip);
-    if (want_clip) {
-        if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g;
-        else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
-    }
-    big_append_wave(ar, big, r, area);
 }
 
 // ---------------------------------------------------------------------------
@@ -1064,8 +1101,8 @@ __global__ __launch_bounds__(256) void k_big(KArgs ka, Arena ar) {
 }
 
 // ---------------------------------------------------------------------------
-// k_resolve: visibility mode's last stage.  One workgroup per bin; bins
-// without keys exit at once.  Every pixel with a key: recompute the winner's
+// k_resolve: visibility mode's last stage.  One wave per bin; bins
+// without keys are skipped at once.  Every pixel with a key: recompute the winner's
 // setup from its face (or read its clipped piece), depth-test against the
 // framebuffer (strict less, reference mergeTileResults), shade, store, and
 // leave the key empty for the next frame.
@@ -1134,10 +1171,10 @@ __device__ __forceinline__ void resolve_vis_pixel(const ViewDesc& V, const Arena
     if (V.ids) V.ids[pix] = key;
 }
 
-template <int T, int NT, class S>
+template <int T, class S>
 __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena& ar, int ox, int oy) {
     unsigned long long* vk = ar.vkeys + V.pix_base;
-    for (int p = threadIdx.x; p < T * T; p += NT) {
+    for (int p = (int)__lane_id(); p < T * T; p += 64) {
         const int px = ox + (p % T), py = oy + (p / T);
         if (px >= V.W || py >= V.H) continue;
         const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
@@ -1148,8 +1185,10 @@ __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena&
     }
 }
 
-template <int T, int NT, bool BATCH>
-__global__ __launch_bounds__(NT) void k_resolve(KArgs ka, Arena ar) {
+// One wave per bin (four per workgroup, grid-stride over the bins): small
+// bins cost no workgroup dispatch of their own and need no barrier.
+template <int T, bool BATCH>
+__global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
     if (ar.counters[CNT_ERROR]) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // the per-frame counters of this mode were last needed by k_big
@@ -1157,22 +1196,24 @@ __global__ __launch_bounds__(NT) void k_resolve(KArgs ka, Arena ar) {
         ar.counters[CNT_CLIP_RECS] = 0;
         *big_counter(ar) = 0ull;
     }
-    for (uint32_t t = blockIdx.x; t < ka.total_tiles; t += gridDim.x) {
-        if (ar.tile_count[t] == 0u) continue;   // the same word for every thread
-        __syncthreads();
-        if (threadIdx.x == 0) ar.tile_count[t] = 0u;
+    const uint32_t nw = gridDim.x * 4u;
+    for (uint32_t t = (blockIdx.x * 256u + threadIdx.x) >> 6; t < ka.total_tiles; t += nw) {
+        // the same word for the whole wave; the branch waits for the load, so
+        // every lane has read the flag before lane 0 clears it
+        if (ar.tile_count[t] == 0u) continue;
+        if (__lane_id() == 0) ar.tile_count[t] = 0u;
         uint32_t vi = 0;
         if constexpr (BATCH) vi = ka.tile_views[t];
         const ViewDesc& V = view_at<BATCH>(ka, vi);
         const uint32_t local = t - V.tile_base;
         const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
         switch (V.u.kind) {
-        case SHADER_FLAT: resolve_vis_tile<T, NT, FlatShader>(V, ar, ox, oy); break;
-        case SHADER_PHONG: resolve_vis_tile<T, NT, PhongShader>(V, ar, ox, oy); break;
-        case SHADER_PBR: resolve_vis_tile<T, NT, PBRShader>(V, ar, ox, oy); break;
-        case SHADER_GOURAUD: resolve_vis_tile<T, NT, GouraudShader>(V, ar, ox, oy); break;
-        case SHADER_PHONG_POINT: resolve_vis_tile<T, NT, PhongPointShader>(V, ar, ox, oy); break;
-        default: resolve_vis_tile<T, NT, FlatColorShader>(V, ar, ox, oy); break;
+        case SHADER_FLAT: resolve_vis_tile<T, FlatShader>(V, ar, ox, oy); break;
+        case SHADER_PHONG: resolve_vis_tile<T, PhongShader>(V, ar, ox, oy); break;
+        case SHADER_PBR: resolve_vis_tile<T, PBRShader>(V, ar, ox, oy); break;
+        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader>(V, ar, ox, oy); break;
+        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader>(V, ar, ox, oy); break;
+        default: resolve_vis_tile<T, FlatColorShader>(V, ar, ox, oy); break;
         }
     }
 }
@@ -1180,26 +1221,40 @@ __global__ __launch_bounds__(NT) void k_resolve(KArgs ka, Arena ar) {
 // ---------------------------------------------------------------------------
 // framebuffer clear (reference framebuffer.cpp:26-42)
 
-__global__ void k_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+// Planes start 256-byte aligned (hipMalloc, or 256-byte aligned offsets in
+// a server block), so four pixels go per 16-byte store; the tail is scalar.
+__device__ __forceinline__ void clear_planes(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra,
+                                             float d, size_t first, size_t stride) {
+    const size_t n4 = n / 4;
+    uint4* c4 = reinterpret_cast<uint4*>(color);
+    float4* d4 = reinterpret_cast<float4*>(depth);
+    uint4* i4 = reinterpret_cast<uint4*>(ids);
+    const uint4 cv = make_uint4(bgra, bgra, bgra, bgra);
+    const float4 dv = make_float4(d, d, d, d);
+    const uint4 iv = make_uint4(~0u, ~0u, ~0u, ~0u);
+    for (size_t i = first; i < n4; i += stride) {
+        c4[i] = cv;
+        d4[i] = dv;
+        if (ids) i4[i] = iv;
+    }
+    for (size_t i = 4 * n4 + first; i < n; i += stride) {
         color[i] = bgra;
         depth[i] = d;
         if (ids) ids[i] = 0xFFFFFFFFu;
     }
 }
 
-// Clear of every view of a batch: one lane per pixel of the concatenated
-// framebuffers (views found through the pixel prefix).
-__global__ void k_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
-                              uint32_t bgra, float d) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        const uint32_t v = find_le(pix_bases, nviews, i);
-        const ViewDesc& V = views[v];
-        const uint32_t p = i - pix_bases[v];
-        V.color[p] = bgra;
-        V.depth[p] = d;
-        if (V.ids) V.ids[p] = 0xFFFFFFFFu;
-    }
+__global__ void k_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d) {
+    clear_planes(color, depth, ids, n, bgra, d, blockIdx.x * (size_t)blockDim.x + threadIdx.x,
+                 (size_t)gridDim.x * blockDim.x);
+}
+
+// Clear of every view of a batch: blockIdx.y picks the view (views
+// [view0, view0 + gridDim.y)), the x blocks stride over its pixels.
+__global__ void k_clear_views(const ViewDesc* views, uint32_t view0, uint32_t bgra, float d) {
+    const ViewDesc& V = views[view0 + blockIdx.y];
+    clear_planes(V.color, V.depth, V.ids, (size_t)V.W * (size_t)V.H, bgra, d, blockIdx.x * (size_t)blockDim.x + threadIdx.x,
+                 (size_t)gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1212,28 +1267,28 @@ struct LaunchShape {
 // ev (optional): 5 events recorded after setup, clip, scan, scatter and
 // raster, on the stream the kernels run on.  Expects a clean arena (bin
 // counts and per-frame counters zero) and leaves it clean.
-constexpr int RESOLVE_MAX_BLOCKS = 32768;
+constexpr int RESOLVE_MAX_BLOCKS = 16384;
 
 // Visibility mode: k_vis, k_clip, k_big, k_resolve.  Events: after k_vis,
 // k_clip, k_big, (nothing), k_resolve.
-template <int T, int NT, bool BATCH>
+template <int T, bool BATCH>
 static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
-    const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
-    if (setup_blocks) k_vis<T, BATCH><<<setup_blocks, 256, 0, st>>>(ka, ar);
+    const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
+    if (vis_blocks) k_vis<T, BATCH><<<vis_blocks, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[0], st);
     k_clip<T, BATCH, true><<<256, 64, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[1], st);
     k_big<T, BATCH><<<1024, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[2], st);
     if (ev) (void)hipEventRecord(ev[3], st);
-    k_resolve<T, NT, BATCH><<<imax(1, imin((int)ka.total_tiles, RESOLVE_MAX_BLOCKS)), NT, 0, st>>>(ka, ar);
+    k_resolve<T, BATCH><<<imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[4], st);
     return hipGetLastError();
 }
 
 template <int T, int NT, bool BATCH>
 static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
-    if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, NT, BATCH>(ka, ar, st, ev);
+    if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, BATCH>(ka, ar, st, ev);
     const uint32_t ntiles = ka.total_tiles;
     const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
     if (setup_blocks) k_setup<T, BATCH><<<setup_blocks, 256, 0, st>>>(ka, ar);
@@ -1283,8 +1338,14 @@ hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEve
 
 hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
                               uint32_t bgra, float d, hipStream_t st) {
-    k_clear_views<<<imax(1, imin(4096, (int)((total + 255) / 256))), 256, 0, st>>>(views, pix_bases, nviews, total,
-                                                                                   bgra, d);
+    (void)pix_bases;
+    // about 8 Ki pixels of the average view per block, at most 64 blocks a view
+    const uint32_t avg = nviews ? total / nviews : 0;
+    const int bx = imax(1, imin(64, (int)((avg + 8191) / 8192)));
+    for (uint32_t v0 = 0; v0 < nviews; v0 += 65535u) {
+        const uint32_t nv = nviews - v0 < 65535u ? nviews - v0 : 65535u;
+        k_clear_views<<<dim3(bx, nv), 256, 0, st>>>(views, v0, bgra, d);
+    }
     return hipGetLastError();
 }
 
@@ -1294,7 +1355,7 @@ hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st) {
 }
 
 hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st) {
-    int blocks = (int)imin(2048, (int)((n + 255) / 256));
+    int blocks = (int)imin(2048, (int)((n / 4 + 255) / 256));
     k_clear<<<imax(1, blocks), 256, 0, st>>>(color, depth, ids, n, bgra, d);
     return hipGetLastError();
 }

@@ -67,6 +67,25 @@ AXR_HD vec4 sample(const DevTexture& t, vec2 uv) {
     return t.filter ? sample_bilinear(t, uv) : sample_nearest(t, uv);
 }
 
+// Shading-only math.  Colour is compared with the reference to within one
+// 8-bit step, not bit for bit, so the lighting terms use the hardware's
+// ~1-ulp reciprocal square root, log2 and exp2 (v_rsq_f32, v_log_f32,
+// v_exp_f32) instead of correctly rounded sqrt, divide and powf: their error
+// is about 1e-7 relative, four orders of magnitude below 1/255.  Texture
+// coordinates and barycentrics stay exact (nearest-texel sampling is
+// discontinuous in uv), and so does everything that decides coverage, depth
+// and winners.
+#if defined(__HIP_DEVICE_COMPILE__)
+AXR_HD vec3 fnormalize(vec3 v) { return v * __builtin_amdgcn_rsqf(dot(v, v)); }
+AXR_HD float fpow(float x, float e) {
+    // x >= 0 at every call site; x^0 = 1 (also for x = 0, as powf)
+    return e == 0.0f ? 1.0f : __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+}
+#else
+AXR_HD vec3 fnormalize(vec3 v) { return normalize(v); }
+AXR_HD float fpow(float x, float e) { return powf(x, e); }
+#endif
+
 struct ShadeCtx {
     const Uniforms* u;
     const DevMaterial* m;
@@ -86,7 +105,7 @@ struct FlatShader {
         return Var{u.normal_mat * make3(v.nx, v.ny, v.nz)};
     }
     static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
-        vec3 n = normalize(bary3(bar, t[0].n, t[1].n, t[2].n));
+        vec3 n = fnormalize(bary3(bar, t[0].n, t[1].n, t[2].n));
         float i = sclamp(dot(-c.u->light_dir, n), 0.0f, 1.0f);
         return vec4{1.0f * i, 1.0f * i, 1.0f * i, 1.0f * i};
     }
@@ -98,9 +117,9 @@ AXR_HD TbnVar tbn_vs(const Vertex56& v, const Uniforms& u) {
     TbnVar o;
     o.uv = make2(v.u, v.v);
     o.wp = xyz(u.model * make4(v.px, v.py, v.pz, 1.0f));
-    o.t = normalize(model_dir(u.model, v.tx, v.ty, v.tz));
-    o.b = normalize(model_dir(u.model, v.bx, v.by, v.bz));
-    o.n = normalize(model_dir(u.model, v.nx, v.ny, v.nz));
+    o.t = fnormalize(model_dir(u.model, v.tx, v.ty, v.tz));
+    o.b = fnormalize(model_dir(u.model, v.bx, v.by, v.bz));
+    o.n = fnormalize(model_dir(u.model, v.nx, v.ny, v.nz));
     return o;
 }
 // Interpolated TBN columns: (b.x*M0 + b.y*M1) + b.z*M2, column by column.
@@ -113,15 +132,15 @@ AXR_HD void tbn_interp(vec3 bar, const TbnVar* t, vec3& iT, vec3& iB, vec3& iN) 
 AXR_HD void phong_surface(vec3 bar, const TbnVar* t, const DevMaterial& m, vec3& normal, vec4& albedo, vec3& fragPos) {
     vec2 uv = bary2(bar, t[0].uv, t[1].uv, t[2].uv);
     vec4 nmv = sample(m.bump, uv);
-    vec3 nms = normalize(xyz(nmv) * 2.0f - make3(1.0f, 1.0f, 1.0f));
+    vec3 nms = fnormalize(xyz(nmv) * 2.0f - make3(1.0f, 1.0f, 1.0f));
     vec3 iT, iB, iN;
     tbn_interp(bar, t, iT, iB, iN);
-    vec3 N = normalize(iN);
-    vec3 T = normalize(iT - N * dot(N, iT));
+    vec3 N = fnormalize(iN);
+    vec3 T = fnormalize(iT - N * dot(N, iT));
     vec3 B = cross(N, T);
     mat3 f{{T, B, N}};
     albedo = sample(m.diffuse, uv);
-    normal = normalize(f * nms);
+    normal = fnormalize(f * nms);
     fragPos = bary3(bar, t[0].wp, t[1].wp, t[2].wp);
 }
 
@@ -132,13 +151,13 @@ struct PhongShader {
         vec3 normal, fragPos; vec4 albedo;
         phong_surface(bar, t, *c.m, normal, albedo, fragPos);
         const Uniforms& u = *c.u;
-        vec3 viewDir = normalize(u.camera_pos - fragPos);
+        vec3 viewDir = fnormalize(u.camera_pos - fragPos);
         vec3 lightDir = -u.light_dir;
         vec3 ambient = 0.1f * u.light_color;
         float diff = smax(dot(normal, lightDir), 0.0f);
         vec3 diffuse = diff * u.light_color;
         vec3 reflectDir = reflect(-lightDir, normal);
-        float spec = powf(smax(dot(viewDir, reflectDir), 0.0f), c.m->specular_exponent * 50.0f);
+        float spec = fpow(smax(dot(viewDir, reflectDir), 0.0f), c.m->specular_exponent * 50.0f);
         vec3 specular = (0.5f * spec) * u.light_color;
         return make4(((ambient + diffuse) + specular) * xyz(albedo), 1.0f);
     }
@@ -151,16 +170,16 @@ struct PhongPointShader {
         vec3 normal, fragPos; vec4 albedo;
         phong_surface(bar, t, *c.m, normal, albedo, fragPos);
         const Uniforms& u = *c.u;
-        vec3 viewDir = normalize(u.camera_pos - fragPos);
+        vec3 viewDir = fnormalize(u.camera_pos - fragPos);
         const float e = c.m->specular_exponent * 50.0f;
         vec3 sum = make3(0.0f, 0.0f, 0.0f);
         for (int l = 0; l < u.num_lights; ++l) {
-            vec3 L = normalize(u.light_pos[l] - fragPos);
+            vec3 L = fnormalize(u.light_pos[l] - fragPos);
             vec3 ambient = 0.1f * u.light_colors[l];
             float diff = smax(dot(normal, L), 0.0f);
             vec3 diffuse = diff * u.light_colors[l];
             vec3 reflectDir = reflect(-L, normal);
-            float spec = powf(smax(dot(viewDir, reflectDir), 0.0f), e);
+            float spec = fpow(smax(dot(viewDir, reflectDir), 0.0f), e);
             vec3 specular = (0.5f * spec) * u.light_colors[l];
             sum = sum + ((ambient + diffuse) + specular);
         }
@@ -168,7 +187,7 @@ struct PhongPointShader {
     }
 };
 
-AXR_HD vec3 vpow(vec3 a, float e) { return make3(powf(a.x, e), powf(a.y, e), powf(a.z, e)); }
+AXR_HD vec3 vpow(vec3 a, float e) { return make3(fpow(a.x, e), fpow(a.y, e), fpow(a.z, e)); }
 
 struct PBRShader {
     using Var = TbnVar;
@@ -179,7 +198,7 @@ struct PBRShader {
         const Uniforms& u = *c.u;
         vec2 uv = bary2(bar, t[0].uv, t[1].uv, t[2].uv);
         vec4 nmv = sample(m.bump, uv);
-        vec3 nms = normalize((xyz(nmv) * 2.0f) - 1.0f);
+        vec3 nms = fnormalize((xyz(nmv) * 2.0f) - 1.0f);
         vec3 iT, iB, iN;
         tbn_interp(bar, t, iT, iB, iN);
         mat3 itbn{{iT, iB, iN}};
@@ -192,11 +211,11 @@ struct PBRShader {
         float omm = 1.0f - metallic;
         // The reference re-orthogonalises the frame but lights with the
         // interpolated one (shaders.hpp:316-343); only the latter matters.
-        vec3 normal = normalize(itbn * nms);
+        vec3 normal = fnormalize(itbn * nms);
         vec3 fragPos = bary3(bar, t[0].wp, t[1].wp, t[2].wp);
-        vec3 viewDir = normalize(u.camera_pos - fragPos);
+        vec3 viewDir = fnormalize(u.camera_pos - fragPos);
         vec3 lightDir = -u.light_dir;
-        vec3 halfway = normalize(lightDir + viewDir);
+        vec3 halfway = fnormalize(lightDir + viewDir);
         float NdotH = smax(dot(normal, halfway), 0.0f);
         float NdotV = smax(dot(normal, viewDir), 0.0f);
         float NdotL = smax(dot(normal, lightDir), 0.0f);
@@ -231,7 +250,7 @@ struct PBRShader {
 struct GouraudShader {
     struct Var { float i; vec2 uv; };
     static AXR_HD Var vs(const Vertex56& v, const Uniforms& u) {
-        vec3 N = normalize(model_dir(u.model, v.nx, v.ny, v.nz));
+        vec3 N = fnormalize(model_dir(u.model, v.nx, v.ny, v.nz));
         return Var{sclamp(dot(-u.light_dir, N), 0.0f, 1.0f), make2(v.u, v.v)};
     }
     static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
