@@ -49,6 +49,35 @@ __device__ __forceinline__ uint32_t find_le(const uint32_t* a, uint32_t n, uint3
     return lo;
 }
 
+// Largest i in [0, n) with a[i] <= x (a ascending, a[0] <= x), searched by
+// the whole wave: 64 probes per round, so about log64(n) dependent loads.
+// Every lane of the wave must call it with the same arguments.
+__device__ __forceinline__ uint32_t wave_find_le(const uint32_t* a, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    const uint32_t lane = __lane_id();
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t p = lo + lane * step;
+        const bool le = p < hi && a[p] <= x;
+        const unsigned long long m = __ballot(le);      // a prefix of the lanes (a is ascending)
+        const uint32_t last = 63u - (uint32_t)__clzll((long long)m);
+        lo = lo + last * step;
+        hi = lo + step < hi ? lo + step : hi;
+    }
+    return lo;
+}
+
+// View of face slot g for every lane of a wave (all 64 lanes call it):
+// one 64-way search for the wave's first face, then only lanes past that
+// view's last face search on their own (a wave rarely spans two views).
+__device__ __forceinline__ uint32_t wave_view_of(const uint32_t* bases, uint32_t nviews, uint32_t g, uint32_t total) {
+    uint32_t g0 = __shfl(g, 0);
+    g0 = g0 < total ? g0 : total - 1;
+    const uint32_t v0 = wave_find_le(bases, nviews, g0);
+    if (g >= total) return v0;
+    return g < bases[v0 + 1] ? v0 : find_le(bases, nviews, g);
+}
+
 __device__ __forceinline__ DevGroup group_of(const ViewDesc& V, uint32_t rank) {
     if (V.ngroups == 1) return V.g0;
     uint32_t lo = 0, hi = V.ngroups;
@@ -432,8 +461,8 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
     bool want_clip = false, binnable = false;
     Rect rc{0, 0, 0, 0};
     uint32_t vi = 0;
+    if constexpr (BATCH) vi = wave_view_of(ka.face_bases, ka.nviews, g, ka.total_faces);
     if (active) {
-        if constexpr (BATCH) vi = find_le(ka.face_bases, ka.nviews, g);
         const ViewDesc& V = view_at<BATCH>(ka, vi);
         const uint32_t s = g - V.face_base;
         FaceGeo f;
@@ -490,7 +519,7 @@ __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
         g[k] = blockIdx.x * (256u * F) + k * 256u + threadIdx.x;
         act[k] = g[k] < ka.total_faces;
         vi[k] = 0;
-        if constexpr (BATCH) { if (act[k]) vi[k] = find_le(ka.face_bases, ka.nviews, g[k]); }
+        if constexpr (BATCH) vi[k] = wave_view_of(ka.face_bases, ka.nviews, g[k], ka.total_faces);
         const ViewDesc& V = view_at<BATCH>(ka, vi[k]);
         s[k] = g[k] - V.face_base;
         if (act[k]) {
@@ -1041,24 +1070,6 @@ __global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
     const uint32_t a = (blockIdx.x % NUM_XCD) * chunk + j;
     if (j >= chunk || a >= nactive) return;
     raster_bin<T, NT, BATCH>(ka, ar, ar.active[a], keys, bigq, nbig);
-}
-
-// Largest i in [0, n) with a[i] <= x (a ascending, a[0] <= x), searched by
-// the whole wave: 64 probes per round, so about log64(n) dependent loads.
-// Every lane of the wave must call it with the same arguments.
-__device__ __forceinline__ uint32_t wave_find_le(const uint32_t* a, uint32_t n, uint32_t x) {
-    uint32_t lo = 0, hi = n;
-    const uint32_t lane = __lane_id();
-    while (hi - lo > 1) {
-        const uint32_t step = (hi - lo + 63) / 64;
-        const uint32_t p = lo + lane * step;
-        const bool le = p < hi && a[p] <= x;
-        const unsigned long long m = __ballot(le);      // a prefix of the lanes (a is ascending)
-        const uint32_t last = 63u - (uint32_t)__clzll((long long)m);
-        lo = lo + last * step;
-        hi = lo + step < hi ? lo + step : hi;
-    }
-    return lo;
 }
 
 // ---------------------------------------------------------------------------

@@ -278,6 +278,67 @@ def _has_tuned(workload):
     return AT.load(workload) is not None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_batch(wl, threads=16):
+    """The CPU checker over every view of the batch, one after another
+    (one frame at a time, its thread pool inside each frame)."""
+    import oracle as O
+    from axiomr_amd import resident as H
+    threads = min(threads, os.cpu_count() or 1)
+    rr = O.OracleRenderer(threads)
+    oscenes = {}
+    t0 = time.perf_counter()
+    for i, vs in enumerate(wl.views):
+        sc = wl.scene(i)
+        if vs.mesh not in oscenes:
+            oscenes[vs.mesh] = O.OracleScene(sc)
+        fb = rr.framebuffer(sc.width, sc.height)
+        O.fb_clear(fb, H.BLACK.packed_bgra(), float("inf"))
+        rr.draw(fb, oscenes[vs.mesh], H.params_of(sc))
+        O.fb_destroy(fb)
+    dt = time.perf_counter() - t0
+    for o in oscenes.values():
+        o.close()
+    rr.close()
+    return dict(views_per_s=len(wl.views) / dt, seconds=dt, threads=threads)
+
+
+def cpu_serve(wl, n=24, threads=16):
+    """Per-request CPU render time (the checker, clear + draw) over a sample
+    of the served workload's request mix."""
+    import oracle as O
+    from axiomr_amd import resident as H
+    threads = min(threads, os.cpu_count() or 1)
+    rr = O.OracleRenderer(threads)
+    oscenes = {}
+    times = []
+    for vs in wl.requests(n, seed=99):
+        sc = wl.scene(vs)
+        if vs.mesh not in oscenes:
+            oscenes[vs.mesh] = O.OracleScene(sc)
+        fb = rr.framebuffer(sc.width, sc.height)
+        t0 = time.perf_counter()
+        O.fb_clear(fb, H.BLACK.packed_bgra(), float("inf"))
+        rr.draw(fb, oscenes[vs.mesh], H.params_of(sc))
+        times.append(time.perf_counter() - t0)
+        O.fb_destroy(fb)
+    for o in oscenes.values():
+        o.close()
+    rr.close()
+    t = np.array(times)
+    return dict(requests=n, mean_ms=float(t.mean() * 1e3), p50_ms=float(np.percentile(t, 50) * 1e3),
+                p99_ms=float(np.percentile(t, 99) * 1e3), frames_per_s=float(1.0 / t.mean()), threads=threads)
+
+
 def cpu_baseline(scene, budget_s=12.0, threads=16):
     """The CPU checker (restated reference tiled pipeline with its thread
     pool) on the same frame, repeated until ~budget_s of CPU work."""
@@ -399,6 +460,9 @@ def main():
         log(f"[bench] batch workload: {len(wl.views)} views, {wl.total_faces} faces, {wl.total_pixels} pixels "
             f"({time.perf_counter() - t:.1f}s)")
         b = run_batch(wl, args.batch_reps, args.tile, not args.untuned)
+        cb = None if args.no_cpu else cpu_batch(wl)
+        if cb:
+            log(f"[bench] batch on the CPU checker: {cb['views_per_s']:.1f} views/s ({cb['threads']} threads)")
         log(f"[bench] batch: plain {b['plain_views_per_s']:.0f} views/s, batched {b['batch_views_per_s']:.0f} views/s, "
             f"x{b['speedup']:.2f}; stages {b['stages_ms']}")
         batch = {"workload": f"{len(wl.views)} views of 32 displaced icospheres (5k-500k triangles, log-uniform), "
@@ -409,7 +473,9 @@ def main():
                  "speedup_vs_plain": round(b["speedup"], 3),
                  "batched_gtri_per_s": round(b["faces"] / (b["batch_ms"] * 1e-3) / 1e9, 3),
                  "batch_ms": round(b["batch_ms"], 3), "stages_ms": {k: round(v, 4) for k, v in b["stages_ms"].items()},
-                 "tunables": b["config"], "plain_tunables": b["plain_config"]}
+                 "tunables": b["config"], "plain_tunables": b["plain_config"],
+                 "cpu_views_per_s": round(cb["views_per_s"], 2) if cb else None,
+                 "cpu_threads": cb["threads"] if cb else None}
 
     served = None
     if not args.no_serve:
@@ -419,6 +485,9 @@ def main():
             f"({time.perf_counter() - t:.1f}s)")
         budget = int(args.serve_budget_gb * (1 << 30))
         sv = run_serve(swl, SERVE_RATES, args.serve_seconds, args.serve_lanes, budget, args.tile, not args.untuned)
+        cs = None if args.no_cpu else cpu_serve(swl)
+        if cs:
+            log(f"[bench] serve requests on the CPU checker: mean {cs['mean_ms']:.1f} ms, p99 {cs['p99_ms']:.1f} ms")
         log(f"[bench] serve capacity {sv['capacity_frames_per_s']:.0f} frames/s, plain "
             f"{sv['plain_frames_per_s']:.0f} frames/s, x{sv['speedup']:.2f}")
         st = sv["stats"]
@@ -433,14 +502,21 @@ def main():
                   "fb_peak_bytes": st["fb_peak_bytes"], "fixed_bytes": st["fixed_bytes"],
                   "batches": st["batches"], "max_views_in_batch": st["max_views_in_batch"],
                   "max_lanes_busy": st["max_lanes_busy"], "overflow_retries": st["overflow_retries"],
-                  "tunables": sv["config"]}
+                  "tunables": sv["config"],
+                  "cpu_request_ms": ({"mean": round(cs["mean_ms"], 2), "p50": round(cs["p50_ms"], 2),
+                                      "p99": round(cs["p99_ms"], 2), "requests": cs["requests"],
+                                      "threads": cs["threads"]} if cs else None)}
 
     cpu = None
     if not args.no_cpu:
         c = cpu_baseline(scene, args.cpu_budget)
         log(f"[bench] cpu baseline: {c['value']:.3f} frames/s over {c['frames']} frames, phases {c['phases_s']}")
+        c1 = cpu_baseline(scene, args.cpu_budget / 2, threads=1)
+        log(f"[bench] cpu baseline, 1 thread: {c1['value']:.3f} frames/s over {c1['frames']} frames")
         cpu = {"value": round(c["value"], 4), "unit": "frames/s", "cores": c["threads"], "kind": "port",
                "sample": f"{c['frames']} frames of the same 1080p 1M-triangle scene, median",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+               "single_thread_value": round(c1["value"], 4), "single_thread_frames": c1["frames"],
                "phases_s": {k: round(v, 5) for k, v in c["phases_s"].items()}}
 
     out = {
