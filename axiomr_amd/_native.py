@@ -86,6 +86,19 @@ class AxrServerStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved0"}
 
 
+class AxrObjGroup(C.Structure):
+    _fields_ = [("material", C.c_char * 128), ("first_face", C.c_uint64), ("face_count", C.c_uint64),
+                ("material_index", C.c_uint32)]
+
+
+class AxrObjMaterial(C.Structure):
+    _fields_ = [("name", C.c_char * 128), ("ambient", C.c_float * 3), ("diffuse", C.c_float * 3),
+                ("specular", C.c_float * 3), ("specular_exponent", C.c_float), ("refractive_index", C.c_float),
+                ("dissolve", C.c_float), ("illumination_model", C.c_int32), ("diffuse_map", C.c_char * 512),
+                ("metallic_map", C.c_char * 512), ("bump_map", C.c_char * 512), ("roughness_map", C.c_char * 512),
+                ("ao_map", C.c_char * 512)]
+
+
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 1, 2, 3, 4
 RASTER_BINNED, RASTER_VISIBILITY = 0, 1
 
@@ -134,6 +147,20 @@ SIGNATURES = {
     "axr_server_get_stats": (C.c_int, [_P, C.POINTER(AxrServerStats)]),
     "axr_server_replay": (C.c_int, [_P, C.c_uint32, C.POINTER(AxrServeRequest), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
+    "axr_obj_load": (C.c_int, [C.c_char_p, _PP]),
+    "axr_obj_destroy": (C.c_int, [_P]),
+    "axr_obj_last_error": (C.c_char_p, []),
+    "axr_obj_counts": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]),
+    "axr_obj_arrays": (C.c_int, [_P, _F, _U32]),
+    "axr_obj_get_group": (C.c_int, [_P, C.c_uint32, C.POINTER(AxrObjGroup)]),
+    "axr_obj_get_material": (C.c_int, [_P, C.c_uint32, C.POINTER(AxrObjMaterial)]),
+    "axr_image_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(_U8)]),
+    "axr_image_free": (None, [_U8]),
+    "axr_image_last_error": (C.c_char_p, []),
+    "axr_image_write_png": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _U8, C.c_int]),
+    "axr_texture_load": (C.c_int, [_P, C.c_char_p, C.c_int, _PP]),
+    "axr_fb_write_png": (C.c_int, [_P, C.c_char_p]),
     "axr_gen_uv_sphere": (C.c_int, [C.c_int, C.c_int, C.c_float, _F, _U32]),
     "axr_gen_icosphere": (C.c_int, [C.c_int, C.c_float, C.c_uint64, _F, _U32]),
     "axr_gen_soup": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,

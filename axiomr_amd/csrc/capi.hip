@@ -657,6 +657,37 @@ int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids) {
     return e == hipSuccess ? AXR_OK : hip_fail(ctx->err, e, "framebuffer read");
 }
 
+int axr_texture_load(axr_ctx* ctx, const char* path, int filter, axr_texture** out) {
+    if (!ctx || !out) return AXR_ERR_INVALID;
+    *out = nullptr;
+    int w = 0, h = 0;
+    uint8_t* rgba = nullptr;
+    int r = axr_image_load(path, &w, &h, &rgba);
+    if (r) return fail(ctx->err, r, "texture: %s", axr_image_last_error());
+    r = axr_texture_create(ctx, rgba, w, h, filter, out);
+    axr_image_free(rgba);
+    return r;
+}
+
+int axr_fb_write_png(axr_fb* fb, const char* path) {
+    if (!fb || !path) return AXR_ERR_INVALID;
+    const size_t n = (size_t)fb->w * fb->h;
+    std::vector<uint32_t> bgra(n);
+    int r = axr_fb_read(fb, reinterpret_cast<uint8_t*>(bgra.data()), nullptr, nullptr);
+    if (r) return r;
+    std::vector<uint8_t> rgba(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t p = bgra[i];
+        rgba[4 * i] = (uint8_t)(p >> 16);
+        rgba[4 * i + 1] = (uint8_t)(p >> 8);
+        rgba[4 * i + 2] = (uint8_t)p;
+        rgba[4 * i + 3] = (uint8_t)(p >> 24);
+    }
+    // framebuffer row 0 is the bottom row; the file's first row is the top
+    r = axr_image_write_png(path, fb->w, fb->h, rgba.data(), 1);
+    return r ? fail(fb->ctx->err, r, "png: %s", axr_image_last_error()) : AXR_OK;
+}
+
 // ---- drawMesh ----------------------------------------------------------------
 
 int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_params* p) {

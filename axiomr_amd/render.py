@@ -121,6 +121,12 @@ class Texture:
         ctx.check(ctx._lib.axr_texture_create(ctx.handle, N.u8ptr(rgba), self.width, self.height, filter, C.byref(h)))
         self.handle = h
 
+    @classmethod
+    def from_file(cls, ctx: Context, path: str, filter: int = FILTER_NEAREST) -> "Texture":
+        """reference Texture(const std::string& filePath) (src/texture.cpp:6-35)."""
+        from .ingest import load_image
+        return cls(ctx, load_image(path), filter)
+
     def close(self):
         if self.handle:
             self.ctx._lib.axr_texture_destroy(self.handle)
@@ -174,10 +180,42 @@ class Mesh:
         self.handle = h
         self.materials = list(materials)   # keep textures alive
 
+    @classmethod
+    def from_obj(cls, ctx: Context, path: str, filter: int = FILTER_NEAREST) -> "Mesh":
+        """reference Mesh(const std::string& filePath) (src/mesh.cpp:8-27):
+        the OBJ, its <dir>/<basename>.mtl materials and their textures.  A
+        group whose material the .mtl lacks raises, as the reference's
+        Mesh::getMaterial (.at()) does when drawn; a model without usemtl has
+        no group and draws nothing."""
+        from .ingest import load_obj
+        model = load_obj(path)
+        for g in model.groups:
+            if g.material is None:
+                raise N.AxrError(N.AXR_ERR_INVALID, f"material '{g.materialName}' is not in the .mtl file")
+        cache = {}
+
+        def tex(p):
+            if not p:
+                return None
+            if p not in cache:
+                cache[p] = Texture.from_file(ctx, p, filter)
+            return cache[p]
+
+        mats = [Material(tex(m.diffuseMap), tex(m.metallicMap), tex(m.bumpMap), tex(m.roughnessMap),
+                         tex(m.aoMap), m.specularExponent) for m in model.materials] or [Material()]
+        groups = [MaterialGroup(g.material, g.startIndex, g.faceCount) for g in model.groups] or \
+            [MaterialGroup(0, 0, 0)]
+        mesh = cls(ctx, model.vertices, model.faces, groups, mats)
+        mesh.model = model
+        mesh.textures = list(cache.values())
+        return mesh
+
     def close(self):
         if self.handle:
             self.ctx._lib.axr_mesh_destroy(self.handle)
             self.handle = None
+        for t in getattr(self, "textures", []):
+            t.close()
 
 
 class Framebuffer:
@@ -219,6 +257,11 @@ class Framebuffer:
         self.ctx.check(self.ctx._lib.axr_fb_read(self.handle, N.u8ptr(bgra), N.fptr(depth),
                                                  N.u32ptr(ids) if ids is not None else None))
         return bgra, depth, ids
+
+    def write_png(self, path: str) -> None:
+        """The colour plane as an RGBA PNG, top row first."""
+        import os
+        self.ctx.check(self.ctx._lib.axr_fb_write_png(self.handle, os.fsencode(path)))
 
     def close(self):
         if self.handle:

@@ -262,6 +262,50 @@ int axr_server_get_stats(axr_server* server, axr_server_stats* out);
 int axr_server_replay(axr_server* server, uint32_t n, const axr_serve_request* reqs, const double* arrival_s,
                       double* latency_s);
 
+/* ---- ingestion and image output (host side) ---------------------------- */
+/* reference Mesh(const std::string&) (src/mesh.cpp:8-27, 305-415): a
+ * Wavefront OBJ file, its materials read from <dir>/<basename>.mtl
+ * (src/mesh.cpp:180-215; mtllib is ignored, as there), vertices
+ * deduplicated on (position, uv, normal), polygons fanned, tangents and
+ * bitangents generated per vertex (src/mesh.cpp:222-303).  Faces before the
+ * first usemtl belong to no group. */
+typedef struct axr_obj axr_obj;
+typedef struct axr_obj_group {
+    char material[128];
+    uint64_t first_face, face_count;
+    uint32_t material_index;      /* into axr_obj_get_material order; UINT32_MAX if the
+                                     .mtl has no such material (reference: .at() throws) */
+} axr_obj_group;
+typedef struct axr_obj_material {  /* reference Material (include/mesh.hpp:20-34) */
+    char name[128];
+    float ambient[3], diffuse[3], specular[3];
+    float specular_exponent, refractive_index, dissolve;
+    int32_t illumination_model;
+    /* texture paths (directory of the .mtl + the map_* argument), "" if none:
+     * map_Kd; map_Ks or refl; map_Bump, bump or norm; map_Ns; map_A0 */
+    char diffuse_map[512], metallic_map[512], bump_map[512], roughness_map[512], ao_map[512];
+} axr_obj_material;
+int axr_obj_load(const char* path, axr_obj** out);
+int axr_obj_destroy(axr_obj* obj);
+const char* axr_obj_last_error(void);              /* this thread's last failure */
+int axr_obj_counts(const axr_obj* obj, uint64_t* nverts, uint64_t* nfaces, uint32_t* ngroups,
+                   uint32_t* nmaterials);
+/* nverts x 14 floats (the 56-byte vertex of axr_mesh_create), nfaces x 3 */
+int axr_obj_arrays(const axr_obj* obj, float* verts56, uint32_t* idx);
+int axr_obj_get_group(const axr_obj* obj, uint32_t i, axr_obj_group* out);
+int axr_obj_get_material(const axr_obj* obj, uint32_t i, axr_obj_material* out);  /* sorted by name */
+
+/* reference Texture(const std::string&) (src/texture.cpp:21-35): the decoded
+ * RGBA8 image, top row first (stbi_load with 4 channels).  PNG (all colour
+ * types, 1-16 bit, tRNS, Adam7) and binary PPM/PGM. */
+int axr_image_load(const char* path, int* w, int* h, uint8_t** rgba);  /* free with axr_image_free */
+void axr_image_free(uint8_t* rgba);
+const char* axr_image_last_error(void);            /* this thread's last failure */
+int axr_image_write_png(const char* path, int w, int h, const uint8_t* rgba, int flip_rows);
+int axr_texture_load(axr_ctx* ctx, const char* path, int filter, axr_texture** out);
+/* The framebuffer's colour as an RGBA PNG, top row first. */
+int axr_fb_write_png(axr_fb* fb, const char* path);
+
 /* ---- synthetic scenes (host only; no device work) ---------------------- */
 /* UV sphere of radius r with `stacks` x `slices` quads: (stacks+1)*(slices+1)
  * vertices, 2*stacks*slices faces. */

@@ -1,0 +1,96 @@
+"""A model read from OBJ/MTL/PNG files (reference Mesh(path), Texture(path))
+drawn on the GPU matches the CPU checker on the same loaded arrays, and a
+framebuffer written as PNG reads back as the framebuffer's pixels."""
+import numpy as np
+import pytest
+
+import harness as H
+from axiomr_amd import render as R
+from axiomr_amd import scenes as S
+from axiomr_amd.ingest import load_image, load_obj, write_png
+
+pytestmark = pytest.mark.gpu
+
+
+def write_assets(tmp_path):
+    v, f = S.uv_sphere(40, 48)
+    tex = {"albedo.png": S.noise_texture(128, 128, 7), "nrm.png": S.normal_map(128, 128, 0.7, 8),
+           "stripes.png": S.checker(64, 64, 4)}
+    for name, img in tex.items():
+        write_png(str(tmp_path / name), img)
+    lines = []
+    for p in v:
+        lines.append("v %.9g %.9g %.9g" % (p[0], p[1], p[2]))
+    for p in v:
+        lines.append("vt %.9g %.9g" % (p[3], p[4]))
+    for p in v:
+        lines.append("vn %.9g %.9g %.9g" % (p[5], p[6], p[7]))
+    half = f.shape[0] // 2
+    for k, (a, b, c) in enumerate(f):
+        if k == 0:
+            lines.append("usemtl skin")
+        if k == half:
+            lines.append("usemtl band")
+        lines.append("f %d/%d/%d %d/%d/%d %d/%d/%d" % (a + 1, a + 1, a + 1, b + 1, b + 1, b + 1, c + 1, c + 1, c + 1))
+    (tmp_path / "ball.obj").write_text("\n".join(lines) + "\n")
+    (tmp_path / "ball.mtl").write_text("newmtl skin\nNs 0.5\nmap_Kd albedo.png\nmap_Bump nrm.png\n\n"
+                                       "newmtl band\nNs 0.2\nmap_Kd stripes.png\nbump nrm.png\n")
+    return str(tmp_path / "ball.obj")
+
+
+def scene_of(model, filt):
+    names = {}
+    mats = []
+    for m in model.materials:
+        def tname(p):
+            if not p:
+                return None
+            names.setdefault(p, S.TextureData(load_image(p), filt))
+            return p
+        mats.append(S.MaterialData(diffuse=tname(m.diffuseMap), bump=tname(m.bumpMap),
+                                   specular_exponent=m.specularExponent))
+    return S.SceneData(name="obj", vertices=model.vertices, faces=model.faces, width=320, height=240,
+                       camera_pos=S.orbit_position(2.7, 35.0, 20.0), camera_target=(0.0, 0.0, 0.0), fov=60.0,
+                       shader={"kind": 1, "lightDirection": (0.3, -1.0, -0.5), "lightColor": (0.9, 0.9, 0.9)},
+                       groups=[(g.material, g.startIndex, g.faceCount) for g in model.groups],
+                       materials=mats, textures=names)
+
+
+@pytest.mark.parametrize("filt", [R.FILTER_NEAREST, R.FILTER_BILINEAR])
+def test_obj_model_matches_checker(ctx, tmp_path, filt):
+    path = write_assets(tmp_path)
+    mesh = R.Mesh.from_obj(ctx, path, filter=filt)
+    model = load_obj(path)
+    assert len(model.groups) == 2 and [g.material for g in model.groups] == [1, 0]   # band, skin sorted
+    sc = scene_of(model, filt)
+    fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+    try:
+        fb.clear(H.BLACK)
+        p = H.params_of(sc)
+        ctx.check(ctx._lib.axr_draw(ctx.handle, fb.handle, mesh.handle, R.C.byref(p)))
+        got = fb.read()
+    finally:
+        fb.close()
+        mesh.close()
+    rep = H.compare(got, H.render_oracle(sc))
+    H.assert_parity(rep, f"obj model filter {filt}")
+    assert rep["covered"] > 20_000
+
+
+def test_framebuffer_png_is_the_framebuffer(ctx, tmp_path):
+    PIL = pytest.importorskip("PIL.Image")
+    sc = S.icosphere_scene(n=16, width=200, height=150, tex=64)
+    gs = H.GpuScene(ctx, sc)
+    fb = R.Framebuffer(ctx, sc.width, sc.height)
+    try:
+        fb.clear(R.Color(10, 20, 30, 255))
+        p = H.params_of(sc)
+        ctx.check(ctx._lib.axr_draw(ctx.handle, fb.handle, gs.mesh.handle, R.C.byref(p)))
+        bgra, _, _ = fb.read()
+        fb.write_png(str(tmp_path / "frame.png"))
+    finally:
+        fb.close()
+        gs.close()
+    img = np.asarray(PIL.open(str(tmp_path / "frame.png")).convert("RGBA"))
+    expect = bgra[::-1][..., [2, 1, 0, 3]]          # rows top first, BGRA -> RGBA
+    np.testing.assert_array_equal(img, expect)
