@@ -20,6 +20,7 @@
 #include <array>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -65,6 +66,16 @@ public:
         ctx.check(axr_texture_create(ctx.handle(), rgba, width, height, f, &m_Tex));
         m_W = width;
         m_H = height;
+    }
+    // reference Texture(const std::string& filePath) (src/texture.cpp:6-35):
+    // PNG or binary PPM/PGM, decoded to RGBA8 with row 0 first
+    Texture(Context& ctx, const std::string& filePath, Filter f = Nearest) : m_Ctx(&ctx) {
+        uint8_t* rgba = nullptr;
+        int r = axr_image_load(filePath.c_str(), &m_W, &m_H, &rgba);
+        if (r != AXR_OK) throw Error(r, axr_image_last_error());
+        r = axr_texture_create(ctx.handle(), rgba, m_W, m_H, f, &m_Tex);
+        axr_image_free(rgba);
+        ctx.check(r);
     }
     ~Texture() { if (m_Tex) axr_texture_destroy(m_Tex); }
     Texture(const Texture&) = delete;
@@ -119,6 +130,47 @@ public:
                                   (uint32_t)m.size(), faceColors.empty() ? nullptr : faceColors.data(), &m_Mesh));
         m_Faces = indices.size() / 3;
     }
+    // reference Mesh(const std::string& filePath) (src/mesh.cpp:8-27): the OBJ,
+    // the materials of <dir>/<basename>.mtl and their textures (owned by the
+    // mesh).  A group naming a material the .mtl lacks throws, as the
+    // reference's getMaterial (.at()) does; a file without usemtl draws nothing.
+    Mesh(Context& ctx, const std::string& filePath, Texture::Filter f = Texture::Nearest) {
+        axr_obj* obj = nullptr;
+        int r = axr_obj_load(filePath.c_str(), &obj);
+        if (r != AXR_OK) throw Error(r, axr_obj_last_error());
+        struct Guard { axr_obj* o; ~Guard() { axr_obj_destroy(o); } } guard{obj};
+        uint64_t nv = 0, nf = 0;
+        uint32_t ng = 0, nm = 0;
+        axr_obj_counts(obj, &nv, &nf, &ng, &nm);
+        std::vector<float> v(nv * 14);
+        std::vector<uint32_t> idx(nf * 3);
+        axr_obj_arrays(obj, v.data(), idx.data());
+        std::vector<axr_material> mats;
+        for (uint32_t i = 0; i < nm; ++i) {
+            axr_obj_material om;
+            axr_obj_get_material(obj, i, &om);
+            auto tex = [&](const char* path) -> axr_texture* {
+                if (!path[0]) return nullptr;
+                m_Textures.emplace_back(new Texture(ctx, std::string(path), f));
+                return m_Textures.back()->handle();
+            };
+            mats.push_back(axr_material{tex(om.diffuse_map), tex(om.metallic_map), tex(om.bump_map),
+                                        tex(om.roughness_map), tex(om.ao_map), om.specular_exponent});
+        }
+        std::vector<axr_group> groups;
+        for (uint32_t i = 0; i < ng; ++i) {
+            axr_obj_group og;
+            axr_obj_get_group(obj, i, &og);
+            if (og.material_index == UINT32_MAX)
+                throw Error(AXR_ERR_INVALID, std::string("material '") + og.material + "' is not in the .mtl file");
+            groups.push_back(axr_group{og.material_index, (uint32_t)og.first_face, (uint32_t)og.face_count});
+        }
+        if (mats.empty()) mats.push_back(axr_material{nullptr, nullptr, nullptr, nullptr, nullptr, 1.0f});
+        if (groups.empty()) groups.push_back(axr_group{0, 0, 0});
+        ctx.check(axr_mesh_create(ctx.handle(), v.data(), nv, idx.data(), nf, groups.data(), (uint32_t)groups.size(),
+                                  mats.data(), (uint32_t)mats.size(), nullptr, &m_Mesh));
+        m_Faces = nf;
+    }
     ~Mesh() { if (m_Mesh) axr_mesh_destroy(m_Mesh); }
     Mesh(const Mesh&) = delete;
     Mesh& operator=(const Mesh&) = delete;
@@ -127,6 +179,7 @@ public:
 private:
     axr_mesh* m_Mesh = nullptr;
     size_t m_Faces = 0;
+    std::vector<std::unique_ptr<Texture>> m_Textures;
 };
 
 class Framebuffer {
@@ -152,6 +205,8 @@ public:
     void read(uint8_t* bgra, float* depth, uint32_t* ids = nullptr) const {
         m_Ctx->check(axr_fb_read(m_Fb, bgra, depth, ids));
     }
+    // the colour plane as an RGBA PNG, top row first
+    void writePNG(const std::string& path) const { m_Ctx->check(axr_fb_write_png(m_Fb, path.c_str())); }
     axr_fb* handle() const { return m_Fb; }
     Context& context() const { return *m_Ctx; }
 private:

@@ -4,15 +4,64 @@
 // bilinear 256^2 checker, camera at (0, 0, 3).  Prints the covered pixel
 // count and an FNV-1a hash of the BGRA and depth planes.
 //
+//   example_drop_in --obj model.obj frame.png
+// loads a model the way the reference's Renderer does (Mesh(path), its .mtl
+// and textures; src/renderer.cpp:58-94), draws it with the Phong shader
+// (light (0, -1, 0)) from (0, 0, 3) at 800x600 and writes the frame as PNG.
+//
 //   g++ -std=c++17 -O2 -I include -I axiomr_amd/cpp axiomr_amd/cpp/example_drop_in.cpp \
 //       -L axiomr_amd -laxiomr_amd -Wl,-rpath,<dir of libaxiomr_amd.so> -o example_drop_in
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "axiomr.hpp"
 
+static uint64_t fnv(const std::vector<uint8_t>& bgra, const std::vector<float>& depth, size_t* covered) {
+    uint64_t h = 1469598103934665603ull;
+    auto mixb = [&](const void* p, size_t n) {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    };
+    mixb(bgra.data(), bgra.size());
+    mixb(depth.data(), depth.size() * 4);
+    *covered = 0;
+    for (float d : depth) *covered += std::isfinite(d);
+    return h;
+}
+
+static int draw_obj(const char* path, const char* png) {
+    const int W = 800, H = 600;
+    try {
+        AR::Context ctx(0);
+        AR::Mesh mesh(ctx, path);
+        AR::Framebuffer fb(ctx, W, H);
+        AR::Camera cam({0, 0, 3}, {0, 0, 0}, 60.0f, (float)W / H);
+        cam.setViewport(0, 0, W, H);
+        AR::GpuTiledPipeline pipe(&cam, &fb);
+        AR::PhongShader shader;
+        shader.lightDirection = {0, -1, 0};
+        pipe.setShader(&shader);
+        fb.clearColor(AR::Color{0, 0, 0, 255});
+        fb.clearDepth();
+        pipe.drawMesh(AR::identity(), mesh);
+        fb.writePNG(png);
+        std::vector<uint8_t> bgra(W * H * 4);
+        std::vector<float> depth(W * H);
+        fb.read(bgra.data(), depth.data());
+        size_t covered;
+        const uint64_t h = fnv(bgra, depth, &covered);
+        std::printf("covered %zu hash %016llx\n", covered, (unsigned long long)h);
+    } catch (const AR::Error& e) {
+        std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 3 && std::string(argv[1]) == "--obj") return draw_obj(argv[2], argv[3]);
     int frames = argc > 1 ? std::atoi(argv[1]) : 1;
     const int W = 800, H = 600, stacks = 100, slices = 100;
     std::vector<AR::Vertex> verts((stacks + 1) * (slices + 1));
@@ -41,15 +90,8 @@ int main(int argc, char** argv) {
         std::vector<uint8_t> bgra(W * H * 4);
         std::vector<float> depth(W * H);
         fb.read(bgra.data(), depth.data());
-        uint64_t h = 1469598103934665603ull;
-        auto mixb = [&](const void* p, size_t n) {
-            const uint8_t* b = static_cast<const uint8_t*>(p);
-            for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
-        };
-        mixb(bgra.data(), bgra.size());
-        mixb(depth.data(), depth.size() * 4);
-        size_t covered = 0;
-        for (float d : depth) covered += std::isfinite(d);
+        size_t covered;
+        const uint64_t h = fnv(bgra, depth, &covered);
         std::printf("covered %zu hash %016llx\n", covered, (unsigned long long)h);
     } catch (const AR::Error& e) {
         std::fprintf(stderr, "error %d: %s\n", e.code, e.what());

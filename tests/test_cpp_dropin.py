@@ -40,3 +40,40 @@ def test_example_matches_python_path(ctx):
         h ^= x
         h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
     assert f"{h:016x}" == hexhash
+
+
+@pytest.mark.gpu
+def test_example_draws_an_obj_like_the_python_path(ctx, tmp_path):
+    """The reference Renderer's flow (Mesh(path) with its .mtl and textures,
+    then drawMesh) through the C++ mirror, against the Python mirror."""
+    PIL = pytest.importorskip("PIL.Image")
+    from test_gpu_ingest import write_assets
+    from axiomr_amd import render as R
+    path = write_assets(tmp_path)
+    png = tmp_path / "frame.png"
+    r = subprocess.run([str(B.EXAMPLE_BIN), "--obj", path, str(png)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    covered, _, hexhash = r.stdout.split()[1:4]
+    mesh = R.Mesh.from_obj(ctx, path)
+    fb = R.Framebuffer(ctx, 800, 600)
+    cam = R.Camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), 60.0, 800 / 600)
+    cam.setViewport(0, 0, 800, 600)
+    pipe = R.TiledPipeline(cam, fb)
+    pipe.setShader(R.PhongShader(lightDirection=(0.0, -1.0, 0.0)))
+    try:
+        fb.clearColor(R.Color(0, 0, 0, 255))
+        fb.clearDepth()
+        pipe.drawMesh(np.eye(4, dtype=np.float32).reshape(16), mesh)
+        bgra, depth, _ = fb.read()
+    finally:
+        fb.close()
+        mesh.close()
+    assert int(covered) == int(np.isfinite(depth).sum()) > 10_000
+    h = 1469598103934665603
+    data = np.concatenate([bgra.reshape(-1), depth.view(np.uint8).reshape(-1)])
+    for x in data.tobytes():
+        h ^= x
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    assert f"{h:016x}" == hexhash
+    img = np.asarray(PIL.open(str(png)).convert("RGBA"))
+    np.testing.assert_array_equal(img, bgra[::-1][..., [2, 1, 0, 3]])
