@@ -5,13 +5,13 @@ reference's Pipeline / TiledPipeline / IShader / Mesh / Texture /
 Framebuffer / Camera classes plus the synthetic workloads."""
 from ._native import AxrError, NativeLibraryMissing, lib  # noqa: F401
 from .render import (  # noqa: F401
-    Batch, Camera, Color, Context, FlatColorShader, FlatShader, Framebuffer, GouraudShader, IShader, Material,
-    MaterialGroup, Mesh, PBRShader, PhongPointShader, PhongShader, Texture, TiledPipeline, camera_matrices,
+    Batch, Camera, Color, Context, FlatColorShader, FlatShader, Framebuffer, GouraudShader, IShader, InputState,
+    Material, MaterialGroup, Mesh, PBRShader, PhongPointShader, PhongShader, Texture, TiledPipeline, camera_matrices,
     draw_params,
 )
 
 __all__ = [
     "AxrError", "NativeLibraryMissing", "lib", "Batch", "Camera", "Color", "Context", "FlatColorShader",
-    "FlatShader", "Framebuffer", "GouraudShader", "IShader", "Material", "MaterialGroup", "Mesh", "PBRShader",
+    "FlatShader", "Framebuffer", "GouraudShader", "IShader", "InputState", "Material", "MaterialGroup", "Mesh", "PBRShader",
     "PhongPointShader", "PhongShader", "Texture", "TiledPipeline", "camera_matrices", "draw_params",
 ]

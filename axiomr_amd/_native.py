@@ -99,7 +99,22 @@ class AxrObjMaterial(C.Structure):
                 ("ao_map", C.c_char * 512)]
 
 
+class AxrCameraInput(C.Structure):
+    _fields_ = [("mouse_x", C.c_float), ("mouse_y", C.c_float), ("right_button", C.c_int32), ("keys", C.c_uint32)]
+
+
+class AxrCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("orientation", C.c_float * 4), ("forward", C.c_float * 3),
+                ("right", C.c_float * 3), ("up", C.c_float * 3), ("delta_yaw", C.c_float),
+                ("delta_pitch", C.c_float), ("velocity", C.c_float * 3), ("fov", C.c_float), ("aspect", C.c_float),
+                ("znear", C.c_float), ("zfar", C.c_float), ("mouse_sensitivity", C.c_float),
+                ("last_mouse", C.c_float * 2), ("viewport", C.c_int32 * 4), ("view", C.c_float * 16),
+                ("proj", C.c_float * 16), ("view_proj", C.c_float * 16), ("view_dirty", C.c_int32),
+                ("proj_dirty", C.c_int32)]
+
+
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 1, 2, 3, 4
+KEY_W, KEY_S, KEY_A, KEY_D = 1, 2, 4, 8
 RASTER_BINNED, RASTER_VISIBILITY = 0, 1
 
 _P = C.c_void_p
@@ -170,6 +185,16 @@ SIGNATURES = {
     "axr_gen_normal_map": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_uint64, _U8]),
     "axr_camera_matrices": (C.c_int, [_F, _F, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
                                       _F, _F, _F, _F]),
+    "axr_camera_init": (C.c_int, [C.POINTER(AxrCamera), _F, _F, C.c_float, C.c_float]),
+    "axr_camera_handle_input": (C.c_int, [C.POINTER(AxrCamera), C.POINTER(AxrCameraInput)]),
+    "axr_camera_update": (C.c_int, [C.POINTER(AxrCamera), C.c_float]),
+    "axr_camera_set_position": (C.c_int, [C.POINTER(AxrCamera), _F]),
+    "axr_camera_set_target": (C.c_int, [C.POINTER(AxrCamera), _F]),
+    "axr_camera_set_fov": (C.c_int, [C.POINTER(AxrCamera), C.c_float]),
+    "axr_camera_set_aspect_ratio": (C.c_int, [C.POINTER(AxrCamera), C.c_float]),
+    "axr_camera_set_viewport": (C.c_int, [C.POINTER(AxrCamera), C.c_int, C.c_int, C.c_int, C.c_int]),
+    "axr_camera_viewport_matrix": (C.c_int, [C.POINTER(AxrCamera), _F]),
+    "axr_camera_path": (C.c_int, [C.POINTER(AxrCamera), C.POINTER(AxrCameraInput), _F, C.c_uint64, _F, _F]),
 }
 
 _lib = None

@@ -30,6 +30,7 @@
 namespace AR {
 
 using mat4 = std::array<float, 16>;   // column-major
+struct vec2 { float x, y; };
 struct vec3 { float x, y, z; };
 
 inline mat4 identity() { return mat4{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}; }
@@ -217,27 +218,54 @@ private:
     float m_Depth = __builtin_huge_valf();
 };
 
-// reference Camera (src/camera.cpp:30-81, 185-205), matrices only
+// What the reference's Window reports to Camera::handleInput
+// (src/camera.cpp:100-127).  Camera::handleInput takes this or any type with
+// the same three calls, such as the reference's own Window.
+struct InputState {
+    vec2 mouse{0, 0};
+    bool rightButton = false;
+    std::string keys;                // held keys among "WSAD"
+    bool isMouseButtonDown(int button) const { return button == 1 && rightButton; }
+    vec2 getMousePos() const { return mouse; }
+    bool isKeyDown(char key) const { return keys.find(key) != std::string::npos; }
+};
+
+// reference Camera (include/camera.hpp, src/camera.cpp) over the native
+// axr_camera record.  As in the reference, the view is identity until the
+// first update(), and setFov/setAspectRatio do not rebuild the projection.
 class Camera {
 public:
-    Camera(vec3 position = {0, 0, 0}, vec3 target = {0, 0, -1}, float fov = 60.0f, float aspectRatio = 16.0f / 9.0f)
-        : m_Pos(position), m_Target(target), m_Fov(fov), m_Aspect(aspectRatio) { update(0.0f); }
-    void setViewport(int x, int y, int w, int h) { (void)x; (void)y; m_VpW = w; m_VpH = h; update(0.0f); }
-    void update(float) {
-        float p[3] = {m_Pos.x, m_Pos.y, m_Pos.z}, t[3] = {m_Target.x, m_Target.y, m_Target.z};
-        axr_camera_matrices(p, t, m_Fov, m_Aspect, 0.1f, 100.0f, m_VpW, m_VpH, m_View.data(), m_Proj.data(),
-                            m_ViewProj.data(), m_Viewport.data());
+    Camera(vec3 position = {0, 0, 0}, vec3 target = {0, 0, -1}, float fov = 60.0f, float aspectRatio = 16.0f / 9.0f) {
+        float p[3] = {position.x, position.y, position.z}, t[3] = {target.x, target.y, target.z};
+        if (axr_camera_init(&m_C, p, t, fov, aspectRatio) != AXR_OK)
+            throw Error(AXR_ERR_INVALID, "Camera: position and target must be finite");
     }
-    const mat4& getViewProjectionMatrix() const { return m_ViewProj; }
-    const mat4& getViewMatrix() const { return m_View; }
-    const mat4& getProjectionMatrix() const { return m_Proj; }
-    mat4 getViewportMatrix() const { return m_Viewport; }
-    vec3 getPosition() const { return m_Pos; }
+    void setPosition(vec3 p) { float v[3] = {p.x, p.y, p.z}; axr_camera_set_position(&m_C, v); }
+    void setTarget(vec3 t) { float v[3] = {t.x, t.y, t.z}; axr_camera_set_target(&m_C, v); }
+    void setFov(float fov) { axr_camera_set_fov(&m_C, fov); }
+    void setAspectRatio(float a) { axr_camera_set_aspect_ratio(&m_C, a); }
+    void setViewport(int x, int y, int w, int h) { axr_camera_set_viewport(&m_C, x, y, w, h); }
+    template <class Window>
+    void handleInput(const Window& window) {
+        axr_camera_input in{};
+        auto m = window.getMousePos();
+        in.mouse_x = m.x;
+        in.mouse_y = m.y;
+        in.right_button = window.isMouseButtonDown(1) ? 1 : 0;
+        in.keys = (window.isKeyDown('W') ? AXR_KEY_W : 0u) | (window.isKeyDown('S') ? AXR_KEY_S : 0u) |
+                  (window.isKeyDown('A') ? AXR_KEY_A : 0u) | (window.isKeyDown('D') ? AXR_KEY_D : 0u);
+        axr_camera_handle_input(&m_C, &in);
+    }
+    void update(float deltaTime) { axr_camera_update(&m_C, deltaTime); }
+    mat4 getViewProjectionMatrix() const { return load(m_C.view_proj); }
+    mat4 getViewMatrix() const { return load(m_C.view); }
+    mat4 getProjectionMatrix() const { return load(m_C.proj); }
+    mat4 getViewportMatrix() const { mat4 m; axr_camera_viewport_matrix(&m_C, m.data()); return m; }
+    vec3 getPosition() const { return {m_C.position[0], m_C.position[1], m_C.position[2]}; }
+    const axr_camera& state() const { return m_C; }
 private:
-    vec3 m_Pos, m_Target;
-    float m_Fov, m_Aspect;
-    int m_VpW = 800, m_VpH = 600;
-    mat4 m_View{}, m_Proj{}, m_ViewProj{}, m_Viewport{};
+    static mat4 load(const float* f) { mat4 m; std::copy(f, f + 16, m.data()); return m; }
+    axr_camera m_C{};
 };
 
 // reference IShader: the concrete type selects the device shader; the public

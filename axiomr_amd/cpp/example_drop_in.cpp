@@ -9,6 +9,11 @@
 // and textures; src/renderer.cpp:58-94), draws it with the Phong shader
 // (light (0, -1, 0)) from (0, 0, 3) at 800x600 and writes the frame as PNG.
 //
+//   example_drop_in --fly N
+// runs the camera half of the reference's frame loop (handleInput, then
+// update(dt); src/renderer.cpp:122-123) for N frames of a scripted input
+// (no GPU needed) and prints the final position and projection*view.
+//
 //   g++ -std=c++17 -O2 -I include -I axiomr_amd/cpp axiomr_amd/cpp/example_drop_in.cpp \
 //       -L axiomr_amd -laxiomr_amd -Wl,-rpath,<dir of libaxiomr_amd.so> -o example_drop_in
 #include <cmath>
@@ -39,6 +44,7 @@ static int draw_obj(const char* path, const char* png) {
         AR::Framebuffer fb(ctx, W, H);
         AR::Camera cam({0, 0, 3}, {0, 0, 0}, 60.0f, (float)W / H);
         cam.setViewport(0, 0, W, H);
+        cam.update(0.0f);                // the frame loop's update builds the view
         AR::GpuTiledPipeline pipe(&cam, &fb);
         AR::PhongShader shader;
         shader.lightDirection = {0, -1, 0};
@@ -60,8 +66,33 @@ static int draw_obj(const char* path, const char* png) {
     return 0;
 }
 
+// Frame i of the scripted input: the mouse drifts right and up, the right
+// button is held except on every fifth frame, and W, W+D, A cycle.
+static AR::InputState scripted_input(int i) {
+    AR::InputState in;
+    in.mouse = {100.0f + 3.0f * i, 200.0f - 2.0f * i};
+    in.rightButton = (i % 5) != 0;
+    in.keys = (i % 3 == 0) ? "W" : (i % 3 == 1) ? "WD" : "A";
+    return in;
+}
+
+static int fly(int frames) {
+    AR::Camera cam({0, 1, 5}, {0, 0, 0}, 60.0f, 800.0f / 600.0f);
+    cam.setViewport(0, 0, 800, 600);
+    for (int i = 0; i < frames; ++i) {
+        cam.handleInput(scripted_input(i));
+        cam.update(1.0f / 60.0f);
+    }
+    AR::vec3 p = cam.getPosition();
+    std::printf("position %.9g %.9g %.9g\nview_proj", p.x, p.y, p.z);
+    for (float v : cam.getViewProjectionMatrix()) std::printf(" %.9g", v);
+    std::printf("\n");
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc > 3 && std::string(argv[1]) == "--obj") return draw_obj(argv[2], argv[3]);
+    if (argc > 2 && std::string(argv[1]) == "--fly") return fly(std::atoi(argv[2]));
     int frames = argc > 1 ? std::atoi(argv[1]) : 1;
     const int W = 800, H = 600, stacks = 100, slices = 100;
     std::vector<AR::Vertex> verts((stacks + 1) * (slices + 1));
@@ -78,6 +109,7 @@ int main(int argc, char** argv) {
         AR::Framebuffer fb(ctx, W, H);
         AR::Camera cam({0, 0, 3}, {0, 0, 0}, 60.0f, (float)W / H);
         cam.setViewport(0, 0, W, H);
+        cam.update(0.0f);                // the frame loop's update builds the view
         AR::GpuTiledPipeline pipe(&cam, &fb);
         AR::GouraudShader shader;
         shader.lightDirection = {0, -1, 0};

@@ -204,27 +204,54 @@ inline mat4 rotate(const mat4& m, float angle, vec3 axis) {
     return r;
 }
 
-// Quaternion (w, x, y, z) for the camera's orientation (host only).
+// Quaternion (w, x, y, z) for the camera's orientation (host only).  The
+// operation orders follow GLM 0.9.9 (gtc/quaternion.inl, detail/type_quat.inl),
+// the reference's un-vendored math library.
 struct quat { float w, x, y, z; };
 inline quat angle_axis(float angle, vec3 axis) {
     float s = sinf(angle * 0.5f);
     return quat{cosf(angle * 0.5f), axis.x * s, axis.y * s, axis.z * s};
 }
+inline float qdot(quat a, quat b) { return (a.w * b.w + a.x * b.x) + (a.y * b.y + a.z * b.z); }
 inline quat qnormalize(quat q) {
-    float len = sqrtf(((q.x * q.x + q.y * q.y) + q.z * q.z) + q.w * q.w);
+    float len = sqrtf(qdot(q, q));
     if (len <= 0.0f) return quat{1, 0, 0, 0};
     float inv = 1.0f / len;
     return quat{q.w * inv, q.x * inv, q.y * inv, q.z * inv};
 }
 inline quat qinverse(quat q) {
-    float d = ((q.x * q.x + q.y * q.y) + q.z * q.z) + q.w * q.w;
+    float d = qdot(q, q);
     return quat{q.w / d, -q.x / d, -q.y / d, -q.z / d};
+}
+// Hamilton product p*q (rotate by q, then by p), terms summed left to right.
+inline quat qmul(quat p, quat q) {
+    return quat{p.w * q.w - p.x * q.x - p.y * q.y - p.z * q.z,
+                p.w * q.x + p.x * q.w + p.y * q.z - p.z * q.y,
+                p.w * q.y + p.y * q.w + p.z * q.x - p.x * q.z,
+                p.w * q.z + p.z * q.w + p.x * q.y - p.y * q.x};
 }
 inline vec3 qrotate(quat q, vec3 v) {
     vec3 u{q.x, q.y, q.z};
     vec3 uv = cross(u, v);
     vec3 uuv = cross(u, uv);
     return v + ((uv * q.w) + uuv) * 2.0f;
+}
+// Shortest-arc rotation taking unit vector a onto unit vector b (S. Melax,
+// "The Shortest Arc Quaternion", Game Programming Gems 1, 2000), with GLM's
+// handling of the parallel and antiparallel cases (epsilon = FLT_EPSILON).
+inline quat rotation_between(vec3 a, vec3 b) {
+    const float eps = 1.1920928955078125e-7f;
+    float c = dot(a, b);
+    if (c >= 1.0f - eps) return quat{1, 0, 0, 0};
+    if (c < -1.0f + eps) {
+        vec3 axis = cross(make3(0.0f, 0.0f, 1.0f), a);
+        if (dot(axis, axis) < eps) axis = cross(make3(1.0f, 0.0f, 0.0f), a);
+        return angle_axis(3.14159265358979323846f, normalize(axis));
+    }
+    vec3 axis = cross(a, b);
+    float s = sqrtf((1.0f + c) * 2.0f);
+    float invs = 1.0f / s;
+    return quat{s * 0.5f, axis.x * invs, axis.y * invs, axis.z * invs};
 }
 inline mat4 mat4_cast(quat q) {
     float qxx = q.x * q.x, qyy = q.y * q.y, qzz = q.z * q.z;

@@ -1,6 +1,5 @@
-// scene.cpp - synthetic workloads (host only): meshes, textures and the
-// camera matrices of the reference's Camera.  The reference loads meshes and
-// textures from files (src/mesh.cpp:300-415, src/texture.cpp:21-35); the
+// scene.cpp - synthetic workloads (host only): meshes and textures.  The
+// reference loads meshes and textures from files (src/mesh.cpp:300-415, src/texture.cpp:21-35); the
 // workloads here are procedural and seeded, so every generator uses its own
 // integer RNG (splitmix64 seeding a PCG32 stream) and never a std::
 // distribution, whose output differs between standard libraries.
@@ -295,46 +294,6 @@ int axr_gen_normal_map(int w, int h, float min_z, uint64_t seed, uint8_t* rgba) 
         }
         rgba[4 * i + 3] = 255;
     }
-    return AXR_OK;
-}
-
-int axr_camera_matrices(const float pos[3], const float target[3], float fov_deg, float aspect, float znear,
-                        float zfar, int vp_w, int vp_h, float* view16, float* proj16, float* view_proj16,
-                        float* viewport16) {
-    if (!pos || !target) return AXR_ERR_INVALID;
-    // reference src/camera.cpp:30-56: orientation turning -z onto the view
-    // direction; fov clamped to [1, 179], aspect to >= 0.1
-    fov_deg = fov_deg < 1.0f ? 1.0f : (fov_deg > 179.0f ? 179.0f : fov_deg);
-    aspect = aspect < 0.1f ? 0.1f : aspect;
-    vec3 p = make3(pos[0], pos[1], pos[2]);
-    vec3 fwd = make3(0.0f, 0.0f, -1.0f);
-    vec3 dir = normalize(make3(target[0], target[1], target[2]) - p);
-    float d = dot(fwd, dir);
-    quat q;
-    if (std::fabs(d + 1.0f) < 0.000001f) {
-        q = angle_axis(3.14159265358979323846f, make3(0.0f, 1.0f, 0.0f));
-    } else {
-        vec3 axis = cross(fwd, dir);
-        if (dot(axis, axis) < 0.000001f) q = quat{1.0f, 0.0f, 0.0f, 0.0f};
-        else q = angle_axis(std::acos(d), normalize(axis));
-    }
-    q = qnormalize(q);
-    // reference camera.cpp:65-81
-    mat4 view = mat4_cast(qinverse(q)) * translate(identity4(), -p);
-    mat4 proj = perspective(fov_deg * 0.01745329251994329576923690768489f, aspect, znear, zfar);
-    mat4 vp = proj * view;
-    // reference camera.cpp:191-205
-    mat4 vpm = identity4();
-    vpm.c[0].x = vp_w * 0.5f;
-    vpm.c[1].y = -(vp_h * 0.5f);
-    vpm.c[2].z = 0.5f;
-    vpm.c[3].x = 0.0f + vp_w * 0.5f;
-    vpm.c[3].y = 0.0f + vp_h * 0.5f;
-    vpm.c[3].z = 0.5f;
-    if (view16) std::memcpy(view16, &view, 64);
-    if (proj16) std::memcpy(proj16, &proj, 64);
-    if (view_proj16) std::memcpy(view_proj16, &vp, 64);
-    if (viewport16) std::memcpy(viewport16, &vpm, 64);
     return AXR_OK;
 }
 

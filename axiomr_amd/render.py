@@ -8,6 +8,7 @@ caller of ``TiledPipeline::drawMesh`` finds the same calls here:
     ctx = Context()
     fb = Framebuffer(ctx, 800, 600)
     cam = Camera((0, 0, 3), (0, 0, 0), 60.0, 800 / 600)
+    cam.setViewport(0, 0, 800, 600); cam.update(0.0)
     pipe = TiledPipeline(cam, fb)
     pipe.setShader(PhongShader(lightDirection=(0, -1, 0), lightColor=(0.6, 0.6, 0.6)))
     fb.clearColor(Color(0, 0, 0, 255)); fb.clearDepth()
@@ -280,38 +281,109 @@ def camera_matrices(position, target, fov, aspect, near=0.1, far=100.0, vp_w=800
     return out
 
 
+@dataclass
+class InputState:
+    """One frame of what the reference's Window tells Camera::handleInput
+    (src/camera.cpp:100-127): mouse position, right button, W/S/A/D held.
+    Any object with the same three methods (a real window) works as well."""
+    mouse: tuple = (0.0, 0.0)
+    rightButton: bool = False
+    keys: str = ""
+
+    def isMouseButtonDown(self, button: int) -> bool:
+        return bool(self.rightButton) if button == 1 else False
+
+    def getMousePos(self):
+        return self.mouse
+
+    def isKeyDown(self, key: str) -> bool:
+        return key in self.keys
+
+
+def _input_of(window) -> N.AxrCameraInput:
+    mx, my = window.getMousePos()
+    keys = 0
+    for ch, bit in (("W", N.KEY_W), ("S", N.KEY_S), ("A", N.KEY_A), ("D", N.KEY_D)):
+        if window.isKeyDown(ch):
+            keys |= bit
+    return N.AxrCameraInput(float(mx), float(my), 1 if window.isMouseButtonDown(1) else 0, keys)
+
+
 class Camera:
-    """reference include/camera.hpp (input handling left out)."""
+    """reference include/camera.hpp / src/camera.cpp: a fly camera whose
+    state lives in a native axr_camera record.  As in the reference, the view
+    is identity until the first update(), and setFov/setAspectRatio do not
+    rebuild the projection."""
 
     def __init__(self, position=(0.0, 0.0, 0.0), target=(0.0, 0.0, -1.0), fov: float = 60.0,
                  aspectRatio: float = 16.0 / 9.0):
-        self.position = np.asarray(position, np.float32)
-        self.target = np.asarray(target, np.float32)
-        self.fov, self.aspect = fov, aspectRatio
-        self.near, self.far = 0.1, 100.0
-        self.viewport = (0, 0, 800, 600)
-        self._update()
+        self._c = N.AxrCamera()
+        pos = np.asarray(position, np.float32)
+        tgt = np.asarray(target, np.float32)
+        N.check(N.lib().axr_camera_init(C.byref(self._c), N.fptr(pos), N.fptr(tgt), float(fov),
+                                        float(aspectRatio)))
 
-    def _update(self):
-        self.view, self.proj, self.view_proj, self.viewport_mat = camera_matrices(
-            self.position, self.target, self.fov, self.aspect, self.near, self.far, self.viewport[2],
-            self.viewport[3])
+    @property
+    def state(self) -> N.AxrCamera:
+        return self._c
+
+    def setPosition(self, position):
+        N.check(N.lib().axr_camera_set_position(C.byref(self._c), N.fptr(np.asarray(position, np.float32))))
+
+    def setTarget(self, target):
+        N.check(N.lib().axr_camera_set_target(C.byref(self._c), N.fptr(np.asarray(target, np.float32))))
+
+    def setFov(self, fov: float):
+        N.check(N.lib().axr_camera_set_fov(C.byref(self._c), float(fov)))
+
+    def setAspectRatio(self, aspectRatio: float):
+        N.check(N.lib().axr_camera_set_aspect_ratio(C.byref(self._c), float(aspectRatio)))
 
     def setViewport(self, x, y, w, h):
-        self.viewport = (x, y, w, h)
-        self._update()
+        N.check(N.lib().axr_camera_set_viewport(C.byref(self._c), int(x), int(y), int(w), int(h)))
 
-    def update(self, dt: float = 0.0):
-        self._update()
+    def handleInput(self, window):
+        inp = _input_of(window)
+        N.check(N.lib().axr_camera_handle_input(C.byref(self._c), C.byref(inp)))
+
+    def update(self, deltaTime: float = 0.0):
+        N.check(N.lib().axr_camera_update(C.byref(self._c), float(deltaTime)))
+
+    def getViewMatrix(self):
+        return np.array(self._c.view, np.float32)
+
+    def getProjectionMatrix(self):
+        return np.array(self._c.proj, np.float32)
 
     def getViewProjectionMatrix(self):
-        return self.view_proj
+        return np.array(self._c.view_proj, np.float32)
 
     def getViewportMatrix(self):
-        return self.viewport_mat
+        out = np.zeros(16, np.float32)
+        N.check(N.lib().axr_camera_viewport_matrix(C.byref(self._c), N.fptr(out)))
+        return out
 
     def getPosition(self):
-        return self.position
+        return np.array(self._c.position, np.float32)
+
+    def vectors(self):
+        """(forward, right, up) as float32 arrays."""
+        return (np.array(self._c.forward, np.float32), np.array(self._c.right, np.float32),
+                np.array(self._c.up, np.float32))
+
+    def path(self, inputs, dts):
+        """Run the reference frame loop (src/renderer.cpp:122-123: handleInput,
+        then update(dt)) over recorded inputs; returns the (n, 16) view-projection
+        matrices and (n, 3) positions of every frame."""
+        n = len(inputs)
+        arr = (N.AxrCameraInput * max(n, 1))(*[_input_of(w) for w in inputs])
+        dt = np.ascontiguousarray(dts, np.float32)
+        if dt.shape != (n,):
+            raise ValueError("one dt per input")
+        vp = np.zeros((n, 16), np.float32)
+        pos = np.zeros((n, 3), np.float32)
+        N.check(N.lib().axr_camera_path(C.byref(self._c), arr, N.fptr(dt), n, N.fptr(vp), N.fptr(pos)))
+        return vp, pos
 
 
 # ---- shaders (reference include/shaders/shaders.hpp) -------------------------

@@ -22,6 +22,7 @@ def shader_of(scene):
 def camera_of(scene):
     cam = R.Camera(scene.camera_pos, scene.camera_target, scene.fov, scene.width / scene.height)
     cam.setViewport(0, 0, scene.width, scene.height)
+    cam.update(0.0)
     return cam
 
 

@@ -327,10 +327,70 @@ int axr_gen_noise_texture(int w, int h, uint64_t seed, uint8_t* rgba);
 int axr_gen_normal_map(int w, int h, float min_z, uint64_t seed, uint8_t* rgba);
 /* reference Camera (src/camera.cpp:30-81): view, projection (RH, z in
  * [-w, w]) and projection*view, column-major; plus the viewport matrix
- * (camera.cpp:191-205). */
+ * (camera.cpp:191-205).  The matrices of a camera constructed at pos
+ * looking at target, after its first update(). */
 int axr_camera_matrices(const float pos[3], const float target[3], float fov_deg, float aspect, float znear,
                         float zfar, int vp_w, int vp_h, float* view16, float* proj16, float* view_proj16,
                         float* viewport16);
+
+/* ---- fly camera (reference src/camera.cpp, include/camera.hpp) ---------
+ * The reference's Camera reads a Window each frame (handleInput) and then
+ * advances (update(dt)).  Here the window's answers arrive as an explicit
+ * axr_camera_input, so a recorded input sequence replays to the same
+ * camera path on every run; axr_camera_path produces a whole path's
+ * matrices in one call, ready to be drawn as one batch.  The state record
+ * is caller-owned (no allocation); matrices are column-major. */
+
+/* Movement keys held during a frame (reference camera.cpp:116-124). */
+enum { AXR_KEY_W = 1, AXR_KEY_S = 2, AXR_KEY_A = 4, AXR_KEY_D = 8 };
+
+/* What Camera::handleInput asks the Window (camera.cpp:100-127): the mouse
+ * position, whether the right button (button 1) is down, the keys held. */
+typedef struct axr_camera_input {
+    float mouse_x, mouse_y;
+    int32_t right_button;
+    uint32_t keys;                  /* AXR_KEY_* bits */
+} axr_camera_input;
+
+/* The reference Camera's members (camera.hpp:38-72). */
+typedef struct axr_camera {
+    float position[3];
+    float orientation[4];           /* quaternion w, x, y, z */
+    float forward[3], right[3], up[3];
+    float delta_yaw, delta_pitch;   /* mouse deltas times sensitivity, from the last input */
+    float velocity[3];              /* sum of the held keys' directions, unnormalized */
+    float fov, aspect, znear, zfar;
+    float mouse_sensitivity;
+    float last_mouse[2];
+    int32_t viewport[4];            /* x, y, width, height */
+    float view[16], proj[16], view_proj[16];
+    int32_t view_dirty, proj_dirty;
+} axr_camera;
+
+/* Camera::Camera (camera.cpp:30-55): fov clamped to [1, 179], aspect to
+ * >= 0.1; the projection is built, the view stays identity until the first
+ * axr_camera_update, as in the reference. */
+int axr_camera_init(axr_camera* cam, const float position[3], const float target[3], float fov_deg, float aspect);
+/* Camera::handleInput (camera.cpp:100-127). */
+int axr_camera_handle_input(axr_camera* cam, const axr_camera_input* input);
+/* Camera::update (camera.cpp:83-98): rotates only while both mouse deltas
+ * are non-zero, moves by velocity*dt, rebuilds the view if it changed. */
+int axr_camera_update(axr_camera* cam, float dt);
+/* Camera setters (camera.cpp:162-189).  As in the reference, a new fov or
+ * aspect marks the projection stale but no later call rebuilds it, so the
+ * projection keeps the constructor's fov and aspect. */
+int axr_camera_set_position(axr_camera* cam, const float position[3]);
+int axr_camera_set_target(axr_camera* cam, const float target[3]);
+int axr_camera_set_fov(axr_camera* cam, float fov_deg);
+int axr_camera_set_aspect_ratio(axr_camera* cam, float aspect);
+int axr_camera_set_viewport(axr_camera* cam, int x, int y, int width, int height);
+/* Camera::getViewportMatrix (camera.cpp:191-205). */
+int axr_camera_viewport_matrix(const axr_camera* cam, float* viewport16);
+/* n frames of the reference's frame loop (src/renderer.cpp:122-123:
+ * handleInput, then update(dt)), writing each frame's projection*view
+ * (16 floats, may be NULL) and position (3 floats, may be NULL). */
+int axr_camera_path(axr_camera* cam, const axr_camera_input* inputs, const float* dts, uint64_t n,
+                    float* view_proj16n, float* positions3n);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,7 @@ def test_example_draws_an_obj_like_the_python_path(ctx, tmp_path):
     fb = R.Framebuffer(ctx, 800, 600)
     cam = R.Camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), 60.0, 800 / 600)
     cam.setViewport(0, 0, 800, 600)
+    cam.update(0.0)
     pipe = R.TiledPipeline(cam, fb)
     pipe.setShader(R.PhongShader(lightDirection=(0.0, -1.0, 0.0)))
     try:
