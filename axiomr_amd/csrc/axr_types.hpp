@@ -196,6 +196,8 @@ struct KArgs {
                                 // of a bin is walked by one lane
     int32_t mode;               // RasterMode
     uint32_t total_pixels;
+    int32_t shader = -1;        // the shader kind every view uses, -1 if they differ:
+                                // selects a resolve kernel built for that shader alone
     ViewDesc one;
 };
 

@@ -284,6 +284,12 @@ int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const Frame
                 c[CNT_NEED_BIG_ITEMS]);
 }
 
+int uniform_shader(const ViewDesc* views, uint32_t n) {
+    for (uint32_t i = 1; i < n; ++i)
+        if (views[i].u.kind != views[0].u.kind) return -1;
+    return n ? views[0].u.kind : -1;
+}
+
 ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,
                    const axr_draw_params* p, int tile) {
     ViewDesc v{};
@@ -710,6 +716,7 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     ka.small_footprint = ctx->cfg.small_footprint;
     ka.mode = ctx->cfg.raster_mode;
     ka.total_pixels = (uint32_t)((uint64_t)fb->w * (uint64_t)fb->h);
+    ka.shader = p->shader;
     const FrameSize fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
     int r = arena_ensure(ctx, ctx->err, ctx->main, fs);
     if (r) return r;
@@ -803,6 +810,7 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     ka.small_footprint = ctx->cfg.small_footprint;
     ka.mode = ctx->cfg.raster_mode;
     ka.total_pixels = (uint32_t)pixels;
+    ka.shader = uniform_shader(b->views.data(), nviews);
     ka.one = b->views[0];
     b->fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
     if ((r = arena_ensure(ctx, err, ctx->main, b->fs))) { axr_batch_destroy(b); return r; }

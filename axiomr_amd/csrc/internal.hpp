@@ -125,6 +125,8 @@ int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev);
 int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs);
 
 // The uniforms drawMesh sets (reference tiled_pipeline.cpp:148-155).
+// The shader kind all n views use, or -1 if they differ.
+int uniform_shader(const ViewDesc* views, uint32_t n);
 ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,
                    const axr_draw_params* p, int tile);
 bool shader_ok(const axr_draw_params* p);

@@ -1196,9 +1196,37 @@ __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena&
     }
 }
 
+// Shader functor of a shader kind.
+template <int SK> struct ShaderOf;
+template <> struct ShaderOf<SHADER_FLAT> { using type = FlatShader; };
+template <> struct ShaderOf<SHADER_PHONG> { using type = PhongShader; };
+template <> struct ShaderOf<SHADER_PBR> { using type = PBRShader; };
+template <> struct ShaderOf<SHADER_GOURAUD> { using type = GouraudShader; };
+template <> struct ShaderOf<SHADER_PHONG_POINT> { using type = PhongPointShader; };
+template <> struct ShaderOf<SHADER_FLAT_COLOR> { using type = FlatColorShader; };
+
+template <int T, int SK>
+__device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const Arena& ar, int ox, int oy) {
+    if constexpr (SK >= 0) {
+        resolve_vis_tile<T, typename ShaderOf<SK>::type>(V, ar, ox, oy);
+    } else {
+        switch (V.u.kind) {
+        case SHADER_FLAT: resolve_vis_tile<T, FlatShader>(V, ar, ox, oy); break;
+        case SHADER_PHONG: resolve_vis_tile<T, PhongShader>(V, ar, ox, oy); break;
+        case SHADER_PBR: resolve_vis_tile<T, PBRShader>(V, ar, ox, oy); break;
+        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader>(V, ar, ox, oy); break;
+        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader>(V, ar, ox, oy); break;
+        default: resolve_vis_tile<T, FlatColorShader>(V, ar, ox, oy); break;
+        }
+    }
+}
+
 // One wave per bin (four per workgroup, grid-stride over the bins): small
-// bins cost no workgroup dispatch of their own and need no barrier.
-template <int T, bool BATCH>
+// bins cost no workgroup dispatch of their own and need no barrier.  SK >= 0
+// builds the kernel for that shader kind alone (every view uses it), so its
+// register budget is that shader's and not the largest of all six; SK = -1
+// picks the shader per bin.
+template <int T, bool BATCH, int SK>
 __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
     if (ar.counters[CNT_ERROR]) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1218,14 +1246,7 @@ __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
         const ViewDesc& V = view_at<BATCH>(ka, vi);
         const uint32_t local = t - V.tile_base;
         const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
-        switch (V.u.kind) {
-        case SHADER_FLAT: resolve_vis_tile<T, FlatShader>(V, ar, ox, oy); break;
-        case SHADER_PHONG: resolve_vis_tile<T, PhongShader>(V, ar, ox, oy); break;
-        case SHADER_PBR: resolve_vis_tile<T, PBRShader>(V, ar, ox, oy); break;
-        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader>(V, ar, ox, oy); break;
-        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader>(V, ar, ox, oy); break;
-        default: resolve_vis_tile<T, FlatColorShader>(V, ar, ox, oy); break;
-        }
+        resolve_vis_tile_kind<T, SK>(V, ar, ox, oy);
     }
 }
 
@@ -1292,7 +1313,16 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     k_big<T, BATCH><<<1024, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[2], st);
     if (ev) (void)hipEventRecord(ev[3], st);
-    k_resolve<T, BATCH><<<imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
+    const int rb = imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS));
+    switch (ka.shader) {
+    case SHADER_FLAT: k_resolve<T, BATCH, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
+    case SHADER_PHONG: k_resolve<T, BATCH, SHADER_PHONG><<<rb, 256, 0, st>>>(ka, ar); break;
+    case SHADER_PBR: k_resolve<T, BATCH, SHADER_PBR><<<rb, 256, 0, st>>>(ka, ar); break;
+    case SHADER_GOURAUD: k_resolve<T, BATCH, SHADER_GOURAUD><<<rb, 256, 0, st>>>(ka, ar); break;
+    case SHADER_PHONG_POINT: k_resolve<T, BATCH, SHADER_PHONG_POINT><<<rb, 256, 0, st>>>(ka, ar); break;
+    case SHADER_FLAT_COLOR: k_resolve<T, BATCH, SHADER_FLAT_COLOR><<<rb, 256, 0, st>>>(ka, ar); break;
+    default: k_resolve<T, BATCH, -1><<<rb, 256, 0, st>>>(ka, ar); break;
+    }
     if (ev) (void)hipEventRecord(ev[4], st);
     return hipGetLastError();
 }

@@ -150,6 +150,7 @@ public:
         ka.small_footprint = small_footprint_;
         ka.mode = mode_;
         ka.total_pixels = (uint32_t)pixels;
+        ka.shader = uniform_shader(L.h_views, n);
         ka.one = L.h_views[0];
         hipStream_t st = L.fa.stream;
         hipError_t e;
