@@ -975,7 +975,33 @@ __device__ __forceinline__ void resolve_tile(const ViewDesc& V, const Arena& ar,
     }
 }
 
-template <int T, int NT, bool BATCH>
+// Shader functor of a shader kind.
+template <int SK> struct ShaderOf;
+template <> struct ShaderOf<SHADER_FLAT> { using type = FlatShader; };
+template <> struct ShaderOf<SHADER_PHONG> { using type = PhongShader; };
+template <> struct ShaderOf<SHADER_PBR> { using type = PBRShader; };
+template <> struct ShaderOf<SHADER_GOURAUD> { using type = GouraudShader; };
+template <> struct ShaderOf<SHADER_PHONG_POINT> { using type = PhongPointShader; };
+template <> struct ShaderOf<SHADER_FLAT_COLOR> { using type = FlatColorShader; };
+
+template <int T, int NT, int SK>
+__device__ __forceinline__ void resolve_tile_kind(const ViewDesc& V, const Arena& ar, const unsigned long long* keys,
+                                                  int ox, int oy) {
+    if constexpr (SK >= 0) {
+        resolve_tile<T, NT, typename ShaderOf<SK>::type>(V, ar, keys, ox, oy);
+    } else {
+        switch (V.u.kind) {
+        case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy); break;
+        case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy); break;
+        case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy); break;
+        case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy); break;
+        case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy); break;
+        default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy); break;
+        }
+    }
+}
+
+template <int T, int NT, bool BATCH, int SK>
 __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uint32_t t, unsigned long long* keys,
                                            uint32_t* bigq, uint32_t& nbig) {
     uint32_t vi = 0;
@@ -1039,24 +1065,18 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
         __syncthreads();
     }
 
-    switch (V.u.kind) {
-    case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy); break;
-    case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy); break;
-    case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy); break;
-    case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy); break;
-    case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy); break;
-    default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy); break;
-    }
+    resolve_tile_kind<T, NT, SK>(V, ar, keys, ox, oy);
 }
 
 // One workgroup per entry of the list of non-empty bins (the grid covers
 // every bin; blocks past the list exit after one load).  Workgroups are
 // dispatched to the 8 XCDs round-robin, so block ids are permuted to give
 // each XCD a contiguous run of the list: neighbouring bins share triangles,
-// vertices and texels in that XCD's L2.
+// vertices and texels in that XCD's L2.  SK >= 0: built for one shader kind
+// (see k_resolve).
 constexpr uint32_t NUM_XCD = 8;
 
-template <int T, int NT, bool BATCH>
+template <int T, int NT, bool BATCH, int SK>
 __global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
     __shared__ unsigned long long keys[T * T];
     __shared__ uint32_t bigq[NT];
@@ -1069,7 +1089,7 @@ __global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
     const uint32_t j = blockIdx.x / NUM_XCD;
     const uint32_t a = (blockIdx.x % NUM_XCD) * chunk + j;
     if (j >= chunk || a >= nactive) return;
-    raster_bin<T, NT, BATCH>(ka, ar, ar.active[a], keys, bigq, nbig);
+    raster_bin<T, NT, BATCH, SK>(ka, ar, ar.active[a], keys, bigq, nbig);
 }
 
 // ---------------------------------------------------------------------------
@@ -1195,15 +1215,6 @@ __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena&
         resolve_vis_pixel<S>(V, ar, k, px, py, pix);
     }
 }
-
-// Shader functor of a shader kind.
-template <int SK> struct ShaderOf;
-template <> struct ShaderOf<SHADER_FLAT> { using type = FlatShader; };
-template <> struct ShaderOf<SHADER_PHONG> { using type = PhongShader; };
-template <> struct ShaderOf<SHADER_PBR> { using type = PBRShader; };
-template <> struct ShaderOf<SHADER_GOURAUD> { using type = GouraudShader; };
-template <> struct ShaderOf<SHADER_PHONG_POINT> { using type = PhongPointShader; };
-template <> struct ShaderOf<SHADER_FLAT_COLOR> { using type = FlatColorShader; };
 
 template <int T, int SK>
 __device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const Arena& ar, int ox, int oy) {
@@ -1350,7 +1361,15 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
     k_scatter<<<sblocks, 256, 0, st>>>(ar, nwaves);
     if (ev) (void)hipEventRecord(ev[3], st);
     const uint32_t rblocks = ((imax(1, (int)ntiles) + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
-    k_raster<T, NT, BATCH><<<rblocks, NT, 0, st>>>(ka, ar);
+    switch (ka.shader) {
+    case SHADER_FLAT: k_raster<T, NT, BATCH, SHADER_FLAT><<<rblocks, NT, 0, st>>>(ka, ar); break;
+    case SHADER_PHONG: k_raster<T, NT, BATCH, SHADER_PHONG><<<rblocks, NT, 0, st>>>(ka, ar); break;
+    case SHADER_PBR: k_raster<T, NT, BATCH, SHADER_PBR><<<rblocks, NT, 0, st>>>(ka, ar); break;
+    case SHADER_GOURAUD: k_raster<T, NT, BATCH, SHADER_GOURAUD><<<rblocks, NT, 0, st>>>(ka, ar); break;
+    case SHADER_PHONG_POINT: k_raster<T, NT, BATCH, SHADER_PHONG_POINT><<<rblocks, NT, 0, st>>>(ka, ar); break;
+    case SHADER_FLAT_COLOR: k_raster<T, NT, BATCH, SHADER_FLAT_COLOR><<<rblocks, NT, 0, st>>>(ka, ar); break;
+    default: k_raster<T, NT, BATCH, -1><<<rblocks, NT, 0, st>>>(ka, ar); break;
+    }
     if (ev) (void)hipEventRecord(ev[4], st);
     return hipGetLastError();
 }
