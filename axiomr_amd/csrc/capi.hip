@@ -13,6 +13,7 @@ struct axr_batch {
     axr_ctx* ctx;
     uint32_t nviews;
     std::vector<ViewDesc> views;
+    std::vector<axr_fb*> fbs;      // the views' framebuffers (for their pending clears)
     ViewDesc* d_views = nullptr;
     uint32_t* d_face_bases = nullptr;
     uint32_t* d_tile_bases = nullptr;
@@ -282,6 +283,15 @@ int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const Frame
                 "large triangles %u, large-triangle items %u); arena grown, frames since the last sync were not drawn",
                 c[CNT_NEED_CLIP_FACES], c[CNT_NEED_CLIP_RECS], c[CNT_NEED_PAIRS], c[CNT_NEED_BIG_RECS],
                 c[CNT_NEED_BIG_ITEMS]);
+}
+
+int fb_settle(axr_fb* fb) {
+    if (!fb->clear_pending) return AXR_OK;
+    hipError_t e = launch_clear(fb->color, fb->depth, fb->ids, (size_t)fb->w * fb->h, fb->clear_bgra, fb->clear_depth,
+                                fb->ctx->main.stream);
+    if (e != hipSuccess) return hip_fail(fb->ctx->err, e, "framebuffer clear");
+    fb->clear_pending = false;
+    return AXR_OK;
 }
 
 int uniform_shader(const ViewDesc* views, uint32_t n) {
@@ -643,9 +653,12 @@ int axr_fb_destroy(axr_fb* fb) {
 
 int axr_fb_clear(axr_fb* fb, uint32_t bgra, float depth) {
     if (!fb) return AXR_ERR_INVALID;
-    (void)hipSetDevice(fb->ctx->device);
-    hipError_t e = launch_clear(fb->color, fb->depth, fb->ids, (size_t)fb->w * fb->h, bgra, depth, fb->ctx->main.stream);
-    return e == hipSuccess ? AXR_OK : hip_fail(fb->ctx->err, e, "framebuffer clear");
+    // recorded, not written: the next draw in visibility mode writes it as
+    // part of its last pass; any other use of the planes writes it first
+    fb->clear_pending = true;
+    fb->clear_bgra = bgra;
+    fb->clear_depth = depth;
+    return AXR_OK;
 }
 
 int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids) {
@@ -653,6 +666,8 @@ int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids) {
     axr_ctx* ctx = fb->ctx;
     (void)hipSetDevice(ctx->device);
     const size_t n = (size_t)fb->w * fb->h;
+    int r = fb_settle(fb);
+    if (r) return r;
     hipError_t e = hipStreamSynchronize(ctx->main.stream);
     if (e == hipSuccess && bgra) e = hipMemcpy(bgra, fb->color, n * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && depth) e = hipMemcpy(depth, fb->depth, n * 4, hipMemcpyDeviceToHost);
@@ -720,9 +735,20 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     const FrameSize fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
     int r = arena_ensure(ctx, ctx->err, ctx->main, fs);
     if (r) return r;
+    if (fb->clear_pending) {
+        if (ka.mode == RASTER_VISIBILITY) {
+            // the resolve pass writes the clear (k_resolve, ka.clear)
+            ka.clear = 1;
+            ka.clear_bgra = fb->clear_bgra;
+            ka.clear_depth = fb->clear_depth;
+        } else if ((r = fb_settle(fb))) {
+            return r;
+        }
+    }
     const bool async = (p->flags & AXR_DRAW_ASYNC) != 0;
     for (int attempt = 0; attempt < 8; ++attempt) {
         if ((r = ctx_launch(ctx, ka))) return r;
+        fb->clear_pending = false;   // written by this launch, even by a frame that overflows
         if (async) return AXR_OK;
         if ((r = arena_read_counters(ctx->err, ctx->main))) return r;
         if (!ctx->main.host_counters[CNT_ERROR]) return AXR_OK;
@@ -747,6 +773,7 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     b->ctx = ctx;
     b->nviews = nviews;
     b->views.resize(nviews);
+    b->fbs.assign(fbs, fbs + nviews);
     std::vector<uint32_t> fbase(nviews + 1), tbase(nviews + 1), pbase(nviews + 1);
     uint64_t faces = 0, tiles = 0, pixels = 0;
     const int tile = ctx->cfg.tile_size;
@@ -825,6 +852,11 @@ int axr_batch_launch(axr_batch* b) {
     FrameArena& fa = ctx->main;
     int r = arena_ensure(ctx, ctx->err, fa, b->fs);
     if (r) return r;
+    // a pending clear is overwritten by the batch's own clear, else written first
+    for (axr_fb* fb : b->fbs) {
+        if (b->clear) fb->clear_pending = false;
+        else if ((r = fb_settle(fb))) return r;
+    }
     if (ctx->profiling) {   // events need the eager path
         if (b->clear) {
             hipError_t e = launch_clear_views(b->d_views, b->d_pix_bases, b->nviews, b->total_pixels, b->clear_bgra,

@@ -88,6 +88,12 @@ struct axr_fb {
     float* depth = nullptr;
     uint32_t* ids = nullptr;
     size_t bytes = 0;
+    // A clear not yet written: the next visibility-mode draw writes it as
+    // part of its resolve; anything else that reads or draws into the planes
+    // writes it first (axr::fb_settle).
+    bool clear_pending = false;
+    uint32_t clear_bgra = 0;
+    float clear_depth = 0.0f;
 };
 
 namespace axr {
@@ -125,6 +131,8 @@ int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev);
 int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs);
 
 // The uniforms drawMesh sets (reference tiled_pipeline.cpp:148-155).
+// Write a framebuffer's pending clear, if any (stream-ordered).
+int fb_settle(axr_fb* fb);
 // The shader kind all n views use, or -1 if they differ.
 int uniform_shader(const ViewDesc* views, uint32_t n);
 ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,

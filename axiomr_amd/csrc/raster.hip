@@ -1138,10 +1138,12 @@ __global__ __launch_bounds__(256) void k_big(KArgs ka, Arena ar) {
 // framebuffer (strict less, reference mergeTileResults), shade, store, and
 // leave the key empty for the next frame.
 
+// Returns whether the winner passed the depth test against dfb (the
+// framebuffer's depth, or the depth of a clear the draw absorbs) and was
+// written.
 template <class S>
-__device__ __forceinline__ void resolve_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
-                                                  int py, size_t pix) {
-    const float dfb = V.depth[pix];
+__device__ __forceinline__ bool resolve_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
+                                                  int py, size_t pix, float dfb) {
     const uint32_t key = (uint32_t)k;
     const uint32_t s = key >> 3, j = key & 7u;
     const DevGroup G = group_of(V, s);
@@ -1180,7 +1182,7 @@ __device__ __forceinline__ void resolve_vis_pixel(const ViewDesc& V, const Arena
     raster_setup(r, rt);
     float al, be, ga, z;
     cover(rt, px, py, al, be, ga, z);
-    if (!(z < dfb)) return;   // reference mergeTileResults: strict less
+    if (!(z < dfb)) return false;   // reference mergeTileResults: strict less
     ShadeCtx c;
     c.u = &V.u;
     c.m = &V.materials[G.material & REC_MATERIAL_MASK];
@@ -1200,34 +1202,63 @@ __device__ __forceinline__ void resolve_vis_pixel(const ViewDesc& V, const Arena
     V.depth[pix] = z;
     V.color[pix] = pack_bgra(col);
     if (V.ids) V.ids[pix] = key;
+    return true;
+}
+
+// A clear that a plain draw absorbs (its values; on = false: none).
+struct ClearSpec {
+    bool on;
+    uint32_t bgra;
+    float depth;
+};
+
+__device__ __forceinline__ void clear_pixel(const ViewDesc& V, size_t pix, const ClearSpec& cs) {
+    V.color[pix] = cs.bgra;
+    V.depth[pix] = cs.depth;
+    if (V.ids) V.ids[pix] = 0xFFFFFFFFu;
 }
 
 template <int T, class S>
-__device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena& ar, int ox, int oy) {
+__device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena& ar, int ox, int oy,
+                                                 const ClearSpec& cs) {
     unsigned long long* vk = ar.vkeys + V.pix_base;
     for (int p = (int)__lane_id(); p < T * T; p += 64) {
         const int px = ox + (p % T), py = oy + (p / T);
         if (px >= V.W || py >= V.H) continue;
         const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
         const unsigned long long k = vk[pix];
-        if (k == EMPTY_KEY) continue;
-        vk[pix] = EMPTY_KEY;
-        resolve_vis_pixel<S>(V, ar, k, px, py, pix);
+        bool drawn = false;
+        if (k != EMPTY_KEY) {
+            vk[pix] = EMPTY_KEY;
+            drawn = resolve_vis_pixel<S>(V, ar, k, px, py, pix, cs.on ? cs.depth : V.depth[pix]);
+        }
+        if (!drawn && cs.on) clear_pixel(V, pix, cs);
+    }
+}
+
+// The pixels of a bin without keys, when the draw absorbs a clear.
+template <int T>
+__device__ __forceinline__ void clear_vis_tile(const ViewDesc& V, int ox, int oy, const ClearSpec& cs) {
+    for (int p = (int)__lane_id(); p < T * T; p += 64) {
+        const int px = ox + (p % T), py = oy + (p / T);
+        if (px >= V.W || py >= V.H) continue;
+        clear_pixel(V, (size_t)py * (size_t)V.W + (size_t)px, cs);
     }
 }
 
 template <int T, int SK>
-__device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const Arena& ar, int ox, int oy) {
+__device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const Arena& ar, int ox, int oy,
+                                                      const ClearSpec& cs) {
     if constexpr (SK >= 0) {
-        resolve_vis_tile<T, typename ShaderOf<SK>::type>(V, ar, ox, oy);
+        resolve_vis_tile<T, typename ShaderOf<SK>::type>(V, ar, ox, oy, cs);
     } else {
         switch (V.u.kind) {
-        case SHADER_FLAT: resolve_vis_tile<T, FlatShader>(V, ar, ox, oy); break;
-        case SHADER_PHONG: resolve_vis_tile<T, PhongShader>(V, ar, ox, oy); break;
-        case SHADER_PBR: resolve_vis_tile<T, PBRShader>(V, ar, ox, oy); break;
-        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader>(V, ar, ox, oy); break;
-        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader>(V, ar, ox, oy); break;
-        default: resolve_vis_tile<T, FlatColorShader>(V, ar, ox, oy); break;
+        case SHADER_FLAT: resolve_vis_tile<T, FlatShader>(V, ar, ox, oy, cs); break;
+        case SHADER_PHONG: resolve_vis_tile<T, PhongShader>(V, ar, ox, oy, cs); break;
+        case SHADER_PBR: resolve_vis_tile<T, PBRShader>(V, ar, ox, oy, cs); break;
+        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader>(V, ar, ox, oy, cs); break;
+        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader>(V, ar, ox, oy, cs); break;
+        default: resolve_vis_tile<T, FlatColorShader>(V, ar, ox, oy, cs); break;
         }
     }
 }
@@ -1237,10 +1268,19 @@ __device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const A
 // builds the kernel for that shader kind alone (every view uses it), so its
 // register budget is that shader's and not the largest of all six; SK = -1
 // picks the shader per bin.
+//
+// ka.clear (plain draws): the framebuffer's pending clear is written here
+// instead of by a pass of its own: every bin is visited, a pixel no winner
+// is written to takes the clear values, and winners are depth-tested
+// against the clear depth without reading the framebuffer.  If the frame
+// failed (an arena overflow), only the clear is written, as a separate
+// clear pass would have.
 template <int T, bool BATCH, int SK>
 __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
-    if (ar.counters[CNT_ERROR]) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const bool failed = ar.counters[CNT_ERROR] != 0u;
+    const ClearSpec cs{ka.clear != 0, ka.clear_bgra, ka.clear_depth};
+    if (failed && !cs.on) return;
+    if (!failed && blockIdx.x == 0 && threadIdx.x == 0) {
         // the per-frame counters of this mode were last needed by k_big
         ar.counters[CNT_CLIP_FACES] = 0;
         ar.counters[CNT_CLIP_RECS] = 0;
@@ -1250,14 +1290,16 @@ __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
     for (uint32_t t = (blockIdx.x * 256u + threadIdx.x) >> 6; t < ka.total_tiles; t += nw) {
         // the same word for the whole wave; the branch waits for the load, so
         // every lane has read the flag before lane 0 clears it
-        if (ar.tile_count[t] == 0u) continue;
-        if (__lane_id() == 0) ar.tile_count[t] = 0u;
+        const bool keys = !failed && ar.tile_count[t] != 0u;
+        if (!keys && !cs.on) continue;
+        if (keys && __lane_id() == 0) ar.tile_count[t] = 0u;
         uint32_t vi = 0;
         if constexpr (BATCH) vi = ka.tile_views[t];
         const ViewDesc& V = view_at<BATCH>(ka, vi);
         const uint32_t local = t - V.tile_base;
         const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
-        resolve_vis_tile_kind<T, SK>(V, ar, ox, oy);
+        if (keys) resolve_vis_tile_kind<T, SK>(V, ar, ox, oy, cs);
+        else clear_vis_tile<T>(V, ox, oy, cs);
     }
 }
 

@@ -167,7 +167,11 @@ int axr_mesh_destroy(axr_mesh* mesh);
 int axr_fb_create(axr_ctx* ctx, int w, int h, int with_ids, axr_fb** out);
 int axr_fb_destroy(axr_fb* fb);
 /* reference Framebuffer::clearColor/clearDepth (src/framebuffer.cpp:26-42):
- * bgra packed as a<<24 | r<<16 | g<<8 | b.  Enqueued, not synchronous. */
+ * bgra packed as a<<24 | r<<16 | g<<8 | b.  Not synchronous: the clear is
+ * recorded, and the next axr_draw in visibility mode writes it in the same
+ * pass that writes the frame (no pass of its own); a read-back, a binned
+ * draw or a batch writes it first.  The pixels are always as if the clear
+ * had run at this call. */
 int axr_fb_clear(axr_fb* fb, uint32_t bgra, float depth);
 /* Synchronous read-back; any pointer may be null. */
 int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids);

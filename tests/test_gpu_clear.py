@@ -1,0 +1,208 @@
+"""A framebuffer clear is recorded by axr_fb_clear and written by the next
+draw's last pass (visibility mode) or before any other use of the planes
+(read-back, binned draw, batch).  Whatever the order of calls, the pixels
+are those of the clear followed by the draws, as the CPU checker gives them."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import harness as H
+import oracle as O
+from axiomr_amd import _native as N
+from axiomr_amd import render as R
+from axiomr_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+TINT = R.Color(10, 200, 30, 77)
+
+
+def small_scene():
+    return S.icosphere_scene(n=16, width=200, height=150, tex=64)
+
+
+def oracle_sequence(scene, steps):
+    """steps: ("clear", color, depth) or ("draw", params) applied in order."""
+    os_ = O.OracleScene(scene)
+    rr = O.OracleRenderer(0)
+    fb = rr.framebuffer(scene.width, scene.height)
+    try:
+        O.fb_clear(fb, 0, float("inf"))
+        for st in steps:
+            if st[0] == "clear":
+                O.fb_clear(fb, st[1].packed_bgra(), st[2])
+            else:
+                rr.draw(fb, os_, st[1])
+        return O.fb_read(fb, scene.width, scene.height)
+    finally:
+        O.fb_destroy(fb)
+        rr.close()
+        os_.close()
+
+
+def draw(ctx, fb, gs, p):
+    ctx.check(ctx._lib.axr_draw(ctx.handle, fb.handle, gs.mesh.handle, C.byref(p)))
+
+
+def median_depth(scene):
+    d = oracle_sequence(scene, [("draw", H.params_of(scene))])[1]
+    return float(np.median(d[np.isfinite(d)]))
+
+
+def test_clear_colour_and_depth_then_draw(ctx):
+    sc = small_scene()
+    d = median_depth(sc)                      # half the sphere's pixels lie behind the clear
+    gs = H.GpuScene(ctx, sc)
+    fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+    try:
+        p = H.params_of(sc)
+        fb.clear(TINT, d)
+        draw(ctx, fb, gs, p)
+        got = fb.read()
+    finally:
+        fb.close()
+        gs.close()
+    ref = oracle_sequence(sc, [("clear", TINT, d), ("draw", p)])
+    H.assert_parity(H.compare(got, ref), "clear at the median depth")
+    assert (got[1] == np.float32(d)).sum() > 1000 and (got[1] < d).sum() > 1000
+
+
+def test_clear_alone_reads_back(ctx):
+    sc = small_scene()
+    gs = H.GpuScene(ctx, sc)
+    fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+    try:
+        fb.clear(TINT, 0.25)
+        bgra, depth, ids = fb.read()
+        assert (bgra.view(np.uint32) == TINT.packed_bgra()).all() and (depth == np.float32(0.25)).all()
+        assert (ids == 0xFFFFFFFF).all()
+        draw(ctx, fb, gs, H.params_of(sc))
+        fb.clear(H.BLACK)                     # a clear after a draw wipes it
+        bgra, depth, ids = fb.read()
+        assert (bgra.view(np.uint32) == H.BLACK.packed_bgra()).all() and np.isinf(depth).all()
+        assert (ids == 0xFFFFFFFF).all()
+    finally:
+        fb.close()
+        gs.close()
+
+
+def test_last_clear_wins_and_second_draw_sees_the_first(ctx):
+    sc = small_scene()
+    gs = H.GpuScene(ctx, sc)
+    fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+    p1 = H.params_of(sc)
+    a = np.radians(35.0)
+    rot = np.array([[np.cos(a), 0, np.sin(a), 0], [0, 1, 0, 0], [-np.sin(a), 0, np.cos(a), 0], [0, 0, 0, 1]])
+    p2 = H.params_of(sc, model=rot.T.astype(np.float32).reshape(16))   # column-major
+    try:
+        fb.clear(R.Color(1, 2, 3, 4), 0.1)
+        fb.clear(TINT)
+        draw(ctx, fb, gs, p1)
+        draw(ctx, fb, gs, p2)
+        got = fb.read()
+    finally:
+        fb.close()
+        gs.close()
+    ref = oracle_sequence(sc, [("clear", TINT, float("inf")), ("draw", p1), ("draw", p2)])
+    H.assert_parity(H.compare(got, ref), "two draws after one clear")
+
+
+def test_async_draw_after_clear(ctx):
+    sc = small_scene()
+    gs = H.GpuScene(ctx, sc)
+    fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+    try:
+        p = H.params_of(sc, flags=N.AXR_DRAW_ASYNC)
+        fb.clear(TINT)
+        draw(ctx, fb, gs, p)
+        ctx.sync()
+        got = fb.read()
+    finally:
+        fb.close()
+        gs.close()
+    p.flags = 0
+    H.assert_parity(H.compare(got, oracle_sequence(sc, [("clear", TINT, float("inf")), ("draw", p)])), "async")
+
+
+def test_batch_without_its_own_clear_writes_pending_clears(ctx):
+    sc = small_scene()
+    gs = H.GpuScene(ctx, sc)
+    fbs = [R.Framebuffer(ctx, sc.width, sc.height, with_ids=True) for _ in range(3)]
+    params = [H.params_of(sc) for _ in range(3)]
+    batch = R.Batch(ctx, fbs, [gs.mesh] * 3, params)
+    try:
+        batch.set_clear(None)
+        for i, fb in enumerate(fbs):
+            fb.clear(R.Color(10 * i, 20, 30, 255), 1.0 if i == 1 else float("inf"))
+        batch.launch()
+        ctx.sync()
+        for i, fb in enumerate(fbs):
+            ref = oracle_sequence(sc, [("clear", R.Color(10 * i, 20, 30, 255), 1.0 if i == 1 else float("inf")),
+                                       ("draw", params[i])])
+            H.assert_parity(H.compare(fb.read(), ref), f"batch view {i}")
+        # the batch's own clear replaces a pending one
+        batch.set_clear(TINT)
+        fbs[0].clear(R.Color(1, 1, 1, 1), 0.0)
+        batch.launch()
+        ctx.sync()
+        ref = oracle_sequence(sc, [("clear", TINT, float("inf")), ("draw", params[0])])
+        H.assert_parity(H.compare(fbs[0].read(), ref), "batch clear over a pending clear")
+    finally:
+        batch.close()
+        for fb in fbs:
+            fb.close()
+        gs.close()
+
+
+def quad_stack(n=20, width=1920, height=1080):
+    """n screen-filling quads (2n triangles, larger than the view, so they are
+    clipped) at depths 1.0 .. 1.0 + 0.1 (n - 1), flat per-face colours."""
+    v = np.zeros((4 * n, 14), np.float32)
+    f = np.zeros((2 * n, 3), np.uint32)
+    for k in range(n):
+        z = -(1.0 + 0.1 * k)
+        s = 4.0 * -z
+        v[4 * k:4 * k + 4, :3] = [(-s, -s, z), (s, -s, z), (s, s, z), (-s, s, z)]
+        v[4 * k:4 * k + 4, 5:8] = (0.0, 0.0, 1.0)
+        f[2 * k] = (4 * k, 4 * k + 1, 4 * k + 2)
+        f[2 * k + 1] = (4 * k, 4 * k + 2, 4 * k + 3)
+    rng = np.random.default_rng(4)
+    colors = rng.integers(0, 2**32, 2 * n, dtype=np.uint64).astype(np.uint32)
+    return S.SceneData(name="quad_stack", vertices=v, faces=f, width=width, height=height,
+                       camera_pos=(0.0, 0.0, 0.0), camera_target=(0.0, 0.0, -1.0), fov=60.0, shader={"kind": 5},
+                       groups=[(0, 0, f.shape[0])], materials=[S.MaterialData()], face_colors=colors)
+
+
+@pytest.mark.parametrize("mode", [N.RASTER_BINNED, N.RASTER_VISIBILITY])
+def test_overflowing_frame_after_clear(mode):
+    """A fresh context's arena is too small for 40 clipped screen-filling
+    triangles at 1080p: a synchronous draw grows it and redraws; an
+    asynchronous one reports the overflow at sync and leaves the cleared
+    framebuffer, as if its clear had run on its own."""
+    sc = quad_stack()
+    p = H.params_of(sc)
+    ref = oracle_sequence(sc, [("clear", TINT, float("inf")), ("draw", p)])
+    for asynchronous in (False, True):
+        ctx = R.Context(0)
+        try:
+            ctx.set_config(tile_size=16, raster_threads=256, small_footprint=128, raster_mode=mode)
+            gs = H.GpuScene(ctx, sc)
+            fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+            try:
+                fb.clear(TINT)
+                if asynchronous:
+                    q = H.params_of(sc, flags=N.AXR_DRAW_ASYNC)
+                    ctx.check(ctx._lib.axr_draw(ctx.handle, fb.handle, gs.mesh.handle, C.byref(q)))
+                    assert ctx._lib.axr_sync(ctx.handle) == -4          # AXR_ERR_OVERFLOW
+                    bgra, depth, ids = fb.read()
+                    assert (bgra.view(np.uint32) == TINT.packed_bgra()).all() and np.isinf(depth).all()
+                    assert (ids == 0xFFFFFFFF).all()
+                    fb.clear(TINT)
+                draw(ctx, fb, gs, p)                                    # grows (again) if needed, redraws
+                H.assert_parity(H.compare(fb.read(), ref), f"quad stack mode {mode} async {asynchronous}")
+            finally:
+                fb.close()
+                gs.close()
+        finally:
+            ctx.close()
