@@ -258,11 +258,11 @@ static int arena_prep_if_dirty(ErrSink& err, FrameArena& fa) {
     return AXR_OK;
 }
 
-int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev) {
+int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev, hipEvent_t wait_last) {
     int r = arena_prep_if_dirty(err, fa);
     if (r) return r;
     if (ev) (void)hipEventRecord(ev[0], fa.stream);
-    hipError_t e = launch_frame(ka, fa.a, fa.stream, ev ? ev + 1 : nullptr);
+    hipError_t e = launch_frame(ka, fa.a, fa.stream, ev ? ev + 1 : nullptr, wait_last);
     return e == hipSuccess ? AXR_OK : hip_fail(err, e, "kernel launch");
 }
 
@@ -285,8 +285,23 @@ int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const Frame
                 c[CNT_NEED_BIG_ITEMS]);
 }
 
+int ctx_join(axr_ctx* ctx) {
+    if (!ctx->alt.stream) return AXR_OK;
+    hipError_t e = hipEventRecord(ctx->join_ev[1], ctx->alt.stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->main.stream, ctx->join_ev[1], 0);
+    return e == hipSuccess ? AXR_OK : hip_fail(ctx->err, e, "stream order");
+}
+
+int ctx_sync_all(axr_ctx* ctx) {
+    hipError_t e = hipStreamSynchronize(ctx->main.stream);
+    if (e == hipSuccess && ctx->alt.stream) e = hipStreamSynchronize(ctx->alt.stream);
+    return e == hipSuccess ? AXR_OK : hip_fail(ctx->err, e, "hipStreamSynchronize");
+}
+
 int fb_settle(axr_fb* fb) {
     if (!fb->clear_pending) return AXR_OK;
+    int r = ctx_join(fb->ctx);
+    if (r) return r;
     hipError_t e = launch_clear(fb->color, fb->depth, fb->ids, (size_t)fb->w * fb->h, fb->clear_bgra, fb->clear_depth,
                                 fb->ctx->main.stream);
     if (e != hipSuccess) return hip_fail(fb->ctx->err, e, "framebuffer clear");
@@ -406,6 +421,8 @@ int axr_ctx_create(int device, uint64_t hbm_budget_bytes, axr_ctx** out) {
     ctx->cfg.bin_capacity = 128;
     ctx->cfg.small_footprint = 12;
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return AXR_ERR_HIP; }
+    for (auto& e : ctx->join_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { axr_ctx_destroy(ctx); return AXR_ERR_HIP; }
     int r = arena_create(ctx, ctx->err, ctx->main);
     if (r != AXR_OK) { axr_ctx_destroy(ctx); return r; }
     *out = ctx;
@@ -415,7 +432,10 @@ int axr_ctx_create(int device, uint64_t hbm_budget_bytes, axr_ctx** out) {
 int axr_ctx_destroy(axr_ctx* ctx) {
     if (!ctx) return AXR_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
+    arena_destroy(ctx, ctx->alt);
     arena_destroy(ctx, ctx->main);
+    for (auto& e : ctx->join_ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto& ring : ctx->ev)
         for (auto& e : ring)
             if (e) (void)hipEventDestroy(e);
@@ -479,23 +499,27 @@ int axr_ctx_memory(const axr_ctx* ctx, uint64_t* in_use, uint64_t* arena) {
     return AXR_OK;
 }
 
+// After a host wait: an arena whose frames overflowed is grown to what the
+// kernels reported (the current capacity is the floor) and reset.
+static int arena_settle_overflow(axr_ctx* ctx, FrameArena& fa) {
+    int r = arena_read_counters(ctx->err, fa);
+    if (r || !fa.host_counters[CNT_ERROR]) return r;
+    FrameSize fs;
+    fs.faces = fa.rec.bytes / sizeof(TriRec);
+    fs.tiles = fa.tile_count.bytes / 4 - 1;
+    fs.pixels = fa.vkeys.bytes / 8;
+    fs.wave_cap = fa.a.wave_cap;
+    return arena_check_overflow(ctx, ctx->err, fa, fs);
+}
+
 int axr_sync(axr_ctx* ctx) {
     if (!ctx) return AXR_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
-    hipError_t e = hipStreamSynchronize(ctx->main.stream);
-    if (e != hipSuccess) return hip_fail(ctx->err, e, "hipStreamSynchronize");
-    int r = arena_read_counters(ctx->err, ctx->main);
+    int r = ctx_sync_all(ctx);
     if (r) return r;
-    if (ctx->main.host_counters[CNT_ERROR]) {
-        // grow to what the kernels reported; keep the current capacity as the floor
-        FrameSize fs;
-        fs.faces = ctx->main.rec.bytes / sizeof(TriRec);
-        fs.tiles = ctx->main.tile_count.bytes / 4 - 1;
-        fs.pixels = ctx->main.vkeys.bytes / 8;
-        fs.wave_cap = ctx->main.a.wave_cap;
-        return arena_check_overflow(ctx, ctx->err, ctx->main, fs);
-    }
-    return AXR_OK;
+    r = arena_settle_overflow(ctx, ctx->main);
+    const int r2 = ctx->alt.stream ? arena_settle_overflow(ctx, ctx->alt) : AXR_OK;
+    return r ? r : r2;
 }
 
 // ---- resources -------------------------------------------------------------
@@ -522,7 +546,7 @@ int axr_texture_create(axr_ctx* ctx, const uint8_t* rgba, int w, int h, int filt
 int axr_texture_destroy(axr_texture* t) {
     if (!t) return AXR_ERR_INVALID;
     (void)hipSetDevice(t->ctx->device);
-    (void)hipStreamSynchronize(t->ctx->main.stream);
+    (void)ctx_sync_all(t->ctx);
     dev_free(t->ctx, (void*)t->dev.texels, t->bytes);
     delete t;
     return AXR_OK;
@@ -598,7 +622,7 @@ int axr_mesh_create(axr_ctx* ctx, const float* verts56, uint64_t nverts, const u
 int axr_mesh_destroy(axr_mesh* m) {
     if (!m) return AXR_ERR_INVALID;
     (void)hipSetDevice(m->ctx->device);
-    (void)hipStreamSynchronize(m->ctx->main.stream);
+    (void)ctx_sync_all(m->ctx);
     dev_free(m->ctx, m->pos, m->nverts * 12);
     dev_free(m->ctx, m->verts, m->nverts * 56);
     dev_free(m->ctx, m->idx, m->nfaces * 12);
@@ -642,7 +666,7 @@ int axr_fb_create(axr_ctx* ctx, int w, int h, int with_ids, axr_fb** out) {
 int axr_fb_destroy(axr_fb* fb) {
     if (!fb) return AXR_ERR_INVALID;
     (void)hipSetDevice(fb->ctx->device);
-    (void)hipStreamSynchronize(fb->ctx->main.stream);
+    (void)ctx_sync_all(fb->ctx);
     const size_t n = (size_t)fb->w * fb->h;
     dev_free(fb->ctx, fb->color, n * 4);
     dev_free(fb->ctx, fb->depth, n * 4);
@@ -667,8 +691,8 @@ int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids) {
     (void)hipSetDevice(ctx->device);
     const size_t n = (size_t)fb->w * fb->h;
     int r = fb_settle(fb);
-    if (r) return r;
-    hipError_t e = hipStreamSynchronize(ctx->main.stream);
+    if (r || (r = ctx_sync_all(ctx))) return r;
+    hipError_t e = hipSuccess;
     if (e == hipSuccess && bgra) e = hipMemcpy(bgra, fb->color, n * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && depth) e = hipMemcpy(depth, fb->depth, n * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && ids) {
@@ -711,6 +735,30 @@ int axr_fb_write_png(axr_fb* fb, const char* path) {
 
 // ---- drawMesh ----------------------------------------------------------------
 
+// An asynchronous visibility-mode draw on the arena whose turn it is: its
+// setup passes go on that arena's stream at once; its resolve pass waits for
+// everything queued so far on the other stream (among it the previous
+// draw's resolve), so it overlaps the previous frame's resolve with this
+// frame's setup and still writes framebuffers in call order.
+static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const FrameSize& fs) {
+    const int x = ctx->next_arena;
+    FrameArena& fa = x ? ctx->alt : ctx->main;
+    FrameArena& other = x ? ctx->main : ctx->alt;
+    int r;
+    if (x && !ctx->alt.stream && (r = arena_create(ctx, ctx->err, ctx->alt))) return r;
+    if ((r = arena_ensure(ctx, ctx->err, fa, fs))) return r;
+    hipEvent_t wait = nullptr;
+    if (other.stream) {
+        wait = ctx->join_ev[x ? 0 : 1];
+        hipError_t e = hipEventRecord(wait, other.stream);
+        if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
+    }
+    if ((r = arena_launch(ctx->err, fa, ka, nullptr, wait))) return r;
+    fb->clear_pending = false;   // written by this launch, even by a frame that overflows
+    ctx->next_arena = x ^ 1;
+    return AXR_OK;
+}
+
 int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_params* p) {
     if (!ctx) return AXR_ERR_INVALID;
     // reference drawMesh returns silently without a shader/camera/framebuffer
@@ -733,19 +781,18 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     ka.total_pixels = (uint32_t)((uint64_t)fb->w * (uint64_t)fb->h);
     ka.shader = p->shader;
     const FrameSize fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
-    int r = arena_ensure(ctx, ctx->err, ctx->main, fs);
-    if (r) return r;
-    if (fb->clear_pending) {
-        if (ka.mode == RASTER_VISIBILITY) {
-            // the resolve pass writes the clear (k_resolve, ka.clear)
-            ka.clear = 1;
-            ka.clear_bgra = fb->clear_bgra;
-            ka.clear_depth = fb->clear_depth;
-        } else if ((r = fb_settle(fb))) {
-            return r;
-        }
+    if (fb->clear_pending && ka.mode == RASTER_VISIBILITY) {
+        // the resolve pass writes the clear (k_resolve, ka.clear)
+        ka.clear = 1;
+        ka.clear_bgra = fb->clear_bgra;
+        ka.clear_depth = fb->clear_depth;
     }
     const bool async = (p->flags & AXR_DRAW_ASYNC) != 0;
+    if (async && ka.mode == RASTER_VISIBILITY && !ctx->profiling) return draw_pipelined(ctx, fb, ka, fs);
+    int r;
+    if ((fb->clear_pending && !ka.clear && (r = fb_settle(fb))) || (r = ctx_join(ctx)) ||
+        (r = arena_ensure(ctx, ctx->err, ctx->main, fs)))
+        return r;
     for (int attempt = 0; attempt < 8; ++attempt) {
         if ((r = ctx_launch(ctx, ka))) return r;
         fb->clear_pending = false;   // written by this launch, even by a frame that overflows
@@ -857,6 +904,7 @@ int axr_batch_launch(axr_batch* b) {
         if (b->clear) fb->clear_pending = false;
         else if ((r = fb_settle(fb))) return r;
     }
+    if ((r = ctx_join(ctx))) return r;
     if (ctx->profiling) {   // events need the eager path
         if (b->clear) {
             hipError_t e = launch_clear_views(b->d_views, b->d_pix_bases, b->nviews, b->total_pixels, b->clear_bgra,
@@ -905,7 +953,7 @@ int axr_batch_destroy(axr_batch* b) {
     if (!b) return AXR_ERR_INVALID;
     axr_ctx* ctx = b->ctx;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->main.stream);
+    (void)ctx_sync_all(ctx);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     const size_t sz_v = b->nviews * sizeof(ViewDesc), sz_b = (b->nviews + 1) * 4;

@@ -12,7 +12,11 @@
 
 namespace axr {
 
-hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev);
+// wait_last (optional): an event the stream waits for before the pass that
+// writes framebuffers (k_resolve / k_raster); the passes before it only touch
+// the frame arena and the meshes.
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev,
+                        hipEvent_t wait_last = nullptr);
 hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st);
 hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st);
 hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
@@ -53,6 +57,15 @@ struct axr_ctx {
     axr::ErrSink err;
     axr_config cfg{};
     axr::FrameArena main;
+    // A second arena for asynchronous visibility-mode draws: consecutive such
+    // draws alternate between the two, so one frame's setup runs while the
+    // previous frame's resolve does.  Each draw's framebuffer pass waits for
+    // all work already queued on the other arena's stream, and anything else
+    // queued on main first waits for the second stream (axr::ctx_join), so
+    // framebuffers see the calls in order.
+    axr::FrameArena alt;
+    hipEvent_t join_ev[2] = {};   // recorded on main's / alt's stream for the other to wait on
+    int next_arena = 0;           // arena of the next asynchronous visibility-mode draw
     bool profiling = false;
     static constexpr int kEventRing = 64;
     hipEvent_t ev[kEventRing][6] = {};
@@ -125,12 +138,17 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs
 int arena_read_counters(ErrSink& err, FrameArena& fa);
 // k_prep if the arena is fresh or dirty, then the frame's kernels; ev (may
 // be null) receives the 5 stage events.
-int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev);
+int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev, hipEvent_t wait_last = nullptr);
 // After arena_read_counters: AXR_OK, or grow the arena to the reported
 // high-water marks, reset it and return AXR_ERR_OVERFLOW.
 int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs);
 
 // The uniforms drawMesh sets (reference tiled_pipeline.cpp:148-155).
+// Queue on the main stream behind everything queued on the second arena's
+// stream (no host wait).
+int ctx_join(axr_ctx* ctx);
+// Host wait for both streams.
+int ctx_sync_all(axr_ctx* ctx);
 // Write a framebuffer's pending clear, if any (stream-ordered).
 int fb_settle(axr_fb* fb);
 // The shader kind all n views use, or -1 if they differ.

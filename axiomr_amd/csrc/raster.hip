@@ -1357,7 +1357,7 @@ constexpr int RESOLVE_MAX_BLOCKS = 16384;
 // Visibility mode: k_vis, k_clip, k_big, k_resolve.  Events: after k_vis,
 // k_clip, k_big, (nothing), k_resolve.
 template <int T, bool BATCH>
-static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t wait_last) {
     const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
     if (vis_blocks) k_vis<T, BATCH><<<vis_blocks, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[0], st);
@@ -1366,6 +1366,8 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     k_big<T, BATCH><<<1024, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[2], st);
     if (ev) (void)hipEventRecord(ev[3], st);
+    // the framebuffer pass waits for what other streams were asked to do first
+    if (wait_last) (void)hipStreamWaitEvent(st, wait_last, 0);
     const int rb = imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS));
     switch (ka.shader) {
     case SHADER_FLAT: k_resolve<T, BATCH, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
@@ -1381,8 +1383,9 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
 }
 
 template <int T, int NT, bool BATCH>
-static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
-    if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, BATCH>(ka, ar, st, ev);
+static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev,
+                                 hipEvent_t wait_last) {
+    if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, BATCH>(ka, ar, st, ev, wait_last);
     const uint32_t ntiles = ka.total_tiles;
     const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
     if (setup_blocks) k_setup<T, BATCH><<<setup_blocks, 256, 0, st>>>(ka, ar);
@@ -1403,6 +1406,7 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
     k_scatter<<<sblocks, 256, 0, st>>>(ar, nwaves);
     if (ev) (void)hipEventRecord(ev[3], st);
     const uint32_t rblocks = ((imax(1, (int)ntiles) + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
+    if (wait_last) (void)hipStreamWaitEvent(st, wait_last, 0);
     switch (ka.shader) {
     case SHADER_FLAT: k_raster<T, NT, BATCH, SHADER_FLAT><<<rblocks, NT, 0, st>>>(ka, ar); break;
     case SHADER_PHONG: k_raster<T, NT, BATCH, SHADER_PHONG><<<rblocks, NT, 0, st>>>(ka, ar); break;
@@ -1417,25 +1421,25 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
 }
 
 template <int T, bool BATCH>
-static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t w) {
     switch (ka.raster_threads) {
-    case 64: return launch_frame_t<T, 64, BATCH>(ka, ar, st, ev);
-    case 128: return launch_frame_t<T, 128, BATCH>(ka, ar, st, ev);
-    default: return launch_frame_t<T, 256, BATCH>(ka, ar, st, ev);
+    case 64: return launch_frame_t<T, 64, BATCH>(ka, ar, st, ev, w);
+    case 128: return launch_frame_t<T, 128, BATCH>(ka, ar, st, ev, w);
+    default: return launch_frame_t<T, 256, BATCH>(ka, ar, st, ev, w);
     }
 }
 
 template <bool BATCH>
-static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
+static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t w) {
     switch (ka.tile_size) {
-    case 8: return launch_frame_nt<8, BATCH>(ka, ar, st, ev);
-    case 32: return launch_frame_nt<32, BATCH>(ka, ar, st, ev);
-    default: return launch_frame_nt<16, BATCH>(ka, ar, st, ev);
+    case 8: return launch_frame_nt<8, BATCH>(ka, ar, st, ev, w);
+    case 32: return launch_frame_nt<32, BATCH>(ka, ar, st, ev, w);
+    default: return launch_frame_nt<16, BATCH>(ka, ar, st, ev, w);
     }
 }
 
-hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev) {
-    return ka.table ? launch_frame_b<true>(ka, ar, st, ev) : launch_frame_b<false>(ka, ar, st, ev);
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t wait_last) {
+    return ka.table ? launch_frame_b<true>(ka, ar, st, ev, wait_last) : launch_frame_b<false>(ka, ar, st, ev, wait_last);
 }
 
 hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,

@@ -206,3 +206,44 @@ def test_overflowing_frame_after_clear(mode):
                 gs.close()
         finally:
             ctx.close()
+
+
+def rotated(sc, deg, flags=0):
+    a = np.radians(deg)
+    rot = np.array([[np.cos(a), 0, np.sin(a), 0], [0, 1, 0, 0], [-np.sin(a), 0, np.cos(a), 0], [0, 0, 0, 1]])
+    return H.params_of(sc, model=rot.T.astype(np.float32).reshape(16), flags=flags)
+
+
+def test_async_draws_keep_call_order(ctx):
+    """Asynchronous draws alternate between two frame arenas and overlap;
+    draws into one framebuffer must still land in call order: six rotated
+    copies drawn without clears in between (each depth-tested against the
+    ones before), interleaved with draws into a second framebuffer, a
+    synchronous draw and a clear."""
+    sc = small_scene()
+    gs = H.GpuScene(ctx, sc)
+    fa = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+    fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+    steps_a, steps_b = [("clear", TINT, float("inf"))], [("clear", H.BLACK, float("inf"))]
+    try:
+        fa.clear(TINT)
+        fb.clear(H.BLACK)
+        for k in range(6):
+            p = rotated(sc, 17.0 * k, N.AXR_DRAW_ASYNC)
+            draw(ctx, fa, gs, p)
+            steps_a.append(("draw", rotated(sc, 17.0 * k)))
+            if k % 2 == 0:
+                q = rotated(sc, -23.0 * k, N.AXR_DRAW_ASYNC if k != 2 else 0)
+                if k == 4:
+                    fb.clear(R.Color(5, 6, 7, 8), 2.0)
+                    steps_b.append(("clear", R.Color(5, 6, 7, 8), 2.0))
+                draw(ctx, fb, gs, q)
+                steps_b.append(("draw", rotated(sc, -23.0 * k)))
+        ctx.sync()
+        got_a, got_b = fa.read(), fb.read()
+    finally:
+        fa.close()
+        fb.close()
+        gs.close()
+    H.assert_parity(H.compare(got_a, oracle_sequence(sc, steps_a)), "six async draws into one framebuffer")
+    H.assert_parity(H.compare(got_b, oracle_sequence(sc, steps_b)), "interleaved framebuffer")
