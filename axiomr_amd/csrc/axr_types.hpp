@@ -198,7 +198,7 @@ struct KArgs {
     uint32_t total_pixels;
     int32_t shader = -1;        // the shader kind every view uses, -1 if they differ:
                                 // selects a resolve kernel built for that shader alone
-    int32_t clear = 0;          // plain visibility draw: k_resolve also clears the
+    int32_t clear = 0;          // visibility mode: k_resolve also clears every view's
     uint32_t clear_bgra = 0;    // framebuffer to these values (a clear the draw
     float clear_depth = 0.0f;   // absorbs instead of a pass of its own)
     ViewDesc one;

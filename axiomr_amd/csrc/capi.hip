@@ -905,13 +905,21 @@ int axr_batch_launch(axr_batch* b) {
         else if ((r = fb_settle(fb))) return r;
     }
     if ((r = ctx_join(ctx))) return r;
+    // in visibility mode the resolve pass writes the batch's clear
+    KArgs ka = b->ka;
+    const bool absorb = b->clear && ka.mode == RASTER_VISIBILITY;
+    if (absorb) {
+        ka.clear = 1;
+        ka.clear_bgra = b->clear_bgra;
+        ka.clear_depth = b->clear_depth;
+    }
     if (ctx->profiling) {   // events need the eager path
-        if (b->clear) {
+        if (b->clear && !absorb) {
             hipError_t e = launch_clear_views(b->d_views, b->d_pix_bases, b->nviews, b->total_pixels, b->clear_bgra,
                                               b->clear_depth, fa.stream);
             if (e != hipSuccess) return hip_fail(ctx->err, e, "batch clear");
         }
-        return ctx_launch(ctx, b->ka);
+        return ctx_launch(ctx, ka);
     }
     if (fa.dirty) {
         hipError_t e = launch_prep(fa.a, (uint32_t)(fa.tile_count.bytes / 4), fa.stream);
@@ -924,10 +932,10 @@ int axr_batch_launch(axr_batch* b) {
         hipError_t e = hipStreamBeginCapture(fa.stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(ctx->err, e, "hipStreamBeginCapture");
         hipError_t le = hipSuccess;
-        if (b->clear)
+        if (b->clear && !absorb)
             le = launch_clear_views(b->d_views, b->d_pix_bases, b->nviews, b->total_pixels, b->clear_bgra,
                                     b->clear_depth, fa.stream);
-        if (le == hipSuccess) le = launch_frame(b->ka, fa.a, fa.stream, nullptr);
+        if (le == hipSuccess) le = launch_frame(ka, fa.a, fa.stream, nullptr);
         e = hipStreamEndCapture(fa.stream, &b->graph);
         if (le != hipSuccess) return hip_fail(ctx->err, le, "batch capture launch");
         if (e != hipSuccess) return hip_fail(ctx->err, e, "hipStreamEndCapture");

@@ -1205,7 +1205,7 @@ __device__ __forceinline__ bool resolve_vis_pixel(const ViewDesc& V, const Arena
     return true;
 }
 
-// A clear that a plain draw absorbs (its values; on = false: none).
+// A clear that a draw absorbs (its values; on = false: none).
 struct ClearSpec {
     bool on;
     uint32_t bgra;
@@ -1269,7 +1269,8 @@ __device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const A
 // register budget is that shader's and not the largest of all six; SK = -1
 // picks the shader per bin.
 //
-// ka.clear (plain draws): the framebuffer's pending clear is written here
+// ka.clear: the framebuffers' clear (a plain draw's pending clear, a batch's
+// or a server batch's clear, the same values for every view) is written here
 // instead of by a pass of its own: every bin is visited, a pixel no winner
 // is written to takes the clear values, and winners are depth-tested
 // against the clear depth without reading the framebuffer.  If the frame

@@ -153,11 +153,20 @@ public:
         ka.shader = uniform_shader(L.h_views, n);
         ka.one = L.h_views[0];
         hipStream_t st = L.fa.stream;
-        hipError_t e;
+        hipError_t e = hipSuccess;
+        // every request starts from a cleared framebuffer; in visibility mode
+        // the resolve pass writes that clear (k_resolve, ka.clear)
+        const bool absorb = mode_ == RASTER_VISIBILITY;
+        if (absorb) {
+            ka.clear = 1;
+            ka.clear_bgra = cfg_.clear_bgra;
+            ka.clear_depth = __builtin_huge_valf();
+        }
         if (n == 1) {
             // one request: the plain path, no tables
             const ViewDesc& v = L.h_views[0];
-            e = launch_clear(v.color, v.depth, v.ids, (size_t)v.W * v.H, cfg_.clear_bgra, __builtin_huge_valf(), st);
+            if (!absorb)
+                e = launch_clear(v.color, v.depth, v.ids, (size_t)v.W * v.H, cfg_.clear_bgra, __builtin_huge_valf(), st);
         } else {
             uint32_t t = 0;
             for (uint32_t i = 0; i < n; ++i)
@@ -170,7 +179,7 @@ public:
             ka.face_bases = L.d_bases;
             ka.tile_bases = L.d_bases + stride;
             ka.tile_views = L.d_tile_views;
-            if (e == hipSuccess)
+            if (e == hipSuccess && !absorb)
                 e = launch_clear_views(L.d_views, L.d_bases + 2 * stride, n, (uint32_t)pixels, cfg_.clear_bgra,
                                        __builtin_huge_valf(), st);
         }
@@ -188,8 +197,8 @@ public:
         if (e == hipErrorNotReady) return 0;
         if (e != hipSuccess) return hip_fail(err_, e, "server batch");
         if (!L.fa.host_counters[CNT_ERROR]) return 1;
-        // arena overflow: the raster stage did not run, so the cleared
-        // framebuffers are untouched; grow to the reported needs and redraw
+        // arena overflow: the raster stage did not run, so the framebuffers
+        // hold (at most) their clear; grow to the reported needs and redraw
         if (++L.attempts > 8) return fail(err_, AXR_ERR_OVERFLOW, "server batch kept overflowing its arena");
         int r = arena_check_overflow(ctx_, err_, L.fa, L.fs);
         if (r != AXR_ERR_OVERFLOW) return r < 0 ? r : AXR_ERR_OVERFLOW;

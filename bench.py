@@ -52,6 +52,14 @@ def raster_bytes(covered, ntex):
     return covered * (4 + 8 + 4 * ntex)
 
 
+def resolve_bytes(W, H, covered, ntex):
+    """Algorithmic bytes of one k_resolve launch, which also writes the
+    frame's clear: colour + depth of every pixel and one texel per texture
+    per covered pixel (the winners are tested against the clear depth, so
+    no framebuffer read)."""
+    return 8 * W * H + 4 * ntex * covered
+
+
 def setup_bytes(nverts, nfaces):
     """Algorithmic bytes of one k_setup launch: indices + positions."""
     return 12 * nfaces + 12 * nverts
@@ -446,9 +454,12 @@ def main():
     dominant = max(stages, key=stages.get)
     if dominant in ("setup", "vis"):
         kbytes, kname = setup_bytes(nverts, nfaces), "k_" + dominant
+    elif "resolve" in stages:
+        dominant = "resolve"
+        kbytes, kname = resolve_bytes(W, H, r["covered"], ntex), "k_resolve"
     else:
-        dominant = "resolve" if "resolve" in stages else "raster"
-        kbytes, kname = raster_bytes(r["covered"], ntex), "k_" + dominant
+        dominant = "raster"
+        kbytes, kname = raster_bytes(r["covered"], ntex), "k_raster"
     kms = stages[dominant]
     achieved = kbytes / (kms * 1e-3) / 1e9
     frame_bytes = algorithmic_bytes(nverts, nfaces, W, H, r["covered"], ntex)
