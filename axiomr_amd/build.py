@@ -59,6 +59,7 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_example()
+    build_rcp_check()
     if verbose:
         print(f"built {LIB}")
     return LIB
@@ -76,6 +77,20 @@ def build_example() -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return EXAMPLE_BIN
+
+
+RCP_SRC = ROOT / "tests" / "native" / "rcp_check.hip"
+RCP_BIN = BUILD / "rcp_check"
+
+
+def build_rcp_check() -> Path:
+    """The exhaustive GPU check of the correctly rounded reciprocal, built
+    with the library's flags (run by tests/test_gpu_rcp.py)."""
+    cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, str(RCP_SRC), "-o", str(RCP_BIN)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"rcp check build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return RCP_BIN
 
 
 if __name__ == "__main__":

@@ -47,6 +47,25 @@ AXR_HD vec3 xyz(vec4 v) { return vec3{v.x, v.y, v.z}; }
 AXR_HD float smin(float a, float b) { return (b < a) ? b : a; }
 AXR_HD float smax(float a, float b) { return (a < b) ? b : a; }
 AXR_HD float sclamp(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+
+// 1/x, correctly rounded (the same bits as the IEEE division 1.0f / x).  On
+// the device the hardware reciprocal refined by one FMA Newton step gives
+// those bits for every x whose biased exponent lies in [2, 252] (checked on
+// gfx950 for all 4,211,081,216 such floats: tests/native/rcp_check.hip);
+// the rest take the division.  Four such reciprocals per triangle setup
+// cost a quarter of the instructions of the division sequence.
+#if defined(__HIP_DEVICE_COMPILE__)
+AXR_HD float rcp_rn(float x) {
+    const uint32_t ex = (__float_as_uint(x) >> 23) & 0xFFu;
+    if (ex - 2u <= 250u) {
+        const float r = __builtin_amdgcn_rcpf(x);
+        return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+    }
+    return 1.0f / x;
+}
+#else
+AXR_HD float rcp_rn(float x) { return 1.0f / x; }
+#endif
 AXR_HD int imin(int a, int b) { return (b < a) ? b : a; }
 AXR_HD int imax(int a, int b) { return (a < b) ? b : a; }
 // std::min({a,b,c}) / std::max({a,b,c}): scan left to right, keep the first

@@ -165,7 +165,7 @@ __device__ __forceinline__ int classify(const vec4& a, const vec4& b, const vec4
 // Triangle setup: screen position (y up) and ndc z of one corner
 // (reference tiled_pipeline.cpp:59-65).
 __device__ __forceinline__ void to_screen(const vec4& c, int W, int H, float& sx, float& sy, float& z) {
-    float invW = 1.0f / clampW(c.w);
+    float invW = rcp_rn(clampW(c.w));
     z = c.z * invW;
     sx = (((c.x * invW) + 1.0f) * 0.5f) * (float)W;
     sy = (((c.y * invW) + 1.0f) * 0.5f) * (float)H;
@@ -221,7 +221,7 @@ __device__ __forceinline__ bool raster_setup(const TriRec& r, RasterTri& t) {
     t.a0 = a0; t.b0 = b0; t.c0 = c0;
     t.a1 = a1; t.b1 = b1; t.c1 = c1;
     t.a2 = a2; t.b2 = b2; t.c2 = c2;
-    t.inv_area = 1.0f / area;
+    t.inv_area = rcp_rn(area);
     t.z0 = r.b.z; t.z1 = r.b.w; t.z2 = __uint_as_float(r.c.x);
     return true;
 }
