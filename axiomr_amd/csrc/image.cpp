@@ -5,8 +5,8 @@
 // stbi_load(path, ..., STBI_rgb_alpha) (src/texture.cpp:21-35): RGBA8, top
 // row first, alpha 255 where the file has none.  Formats: PNG (colour types
 // gray, RGB, palette, gray+alpha, RGBA; bit depths 1-16, 16-bit samples
-// reduced to their high byte; tRNS; Adam7 interlacing) and binary PPM/PGM
-// (P6/P5, maxval 255).  JPEG and the other stb_image formats are not
+// reduced to their high byte; tRNS; Adam7 interlacing), JPEG (jpeg.cpp) and
+// binary PPM/PGM (P6/P5, maxval 255).  The other stb_image formats are not
 // decoded (AXR_ERR_INVALID).
 //
 // Output: PNG, RGBA8, no filter, zlib level 6.  A framebuffer (BGRA, row 0 at
@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/axiomr.h"
+#include "jpeg.hpp"
 
 namespace {
 
@@ -330,6 +331,10 @@ int axr_image_load(const char* path, int* w, int* h, uint8_t** rgba) {
     int r;
     if (f.size() >= 8 && f[0] == 137 && f[1] == 'P' && f[2] == 'N' && f[3] == 'G') r = decode_png(f, w, h, px);
     else if (f.size() >= 2 && f[0] == 'P' && (f[1] == '6' || f[1] == '5')) r = decode_pnm(f, w, h, px);
+    else if (f.size() >= 3 && f[0] == 0xFF && f[1] == 0xD8 && f[2] == 0xFF) {
+        std::string e;
+        r = axr::decode_jpeg(f.data(), f.size(), w, h, px, e) ? AXR_OK : fail(AXR_ERR_INVALID, "JPEG: " + e);
+    }
     else r = fail(AXR_ERR_INVALID, std::string("unsupported image format: ") + path);
     if (r) return r;
     *rgba = (uint8_t*)std::malloc(px.size());

@@ -301,7 +301,8 @@ int axr_obj_get_material(const axr_obj* obj, uint32_t i, axr_obj_material* out);
 
 /* reference Texture(const std::string&) (src/texture.cpp:21-35): the decoded
  * RGBA8 image, top row first (stbi_load with 4 channels).  PNG (all colour
- * types, 1-16 bit, tRNS, Adam7) and binary PPM/PGM. */
+ * types, 1-16 bit, tRNS, Adam7), JPEG (baseline and progressive, the bytes
+ * stb_image gives) and binary PPM/PGM. */
 int axr_image_load(const char* path, int* w, int* h, uint8_t** rgba);  /* free with axr_image_free */
 void axr_image_free(uint8_t* rgba);
 const char* axr_image_last_error(void);            /* this thread's last failure */

@@ -69,12 +69,56 @@ _SIGS = {
 _lib = None
 
 
+REFERENCE = Path("/root/reference")
+STB_PATH = HERE / "_ref" / "libstb_image.so"
+
+
 def build() -> Path:
-    """Compile the restatement with its Makefile (g++ -O3 -mavx2 -ffp-contract=off)."""
+    """Compile the restatement with its Makefile (g++ -O3 -mavx2 -ffp-contract=off),
+    and, where the reference tree is present, its stb_image into _ref/."""
     r = subprocess.run(["make", "-s", "-C", str(HERE)], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    if (REFERENCE / "external" / "stb_image" / "stb_image.cpp").exists():
+        r = subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REFERENCE={REFERENCE}"], capture_output=True,
+                           text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference stb_image build failed:\n{r.stdout}\n{r.stderr}")
     return LIB_PATH
+
+
+_stb = None
+
+
+def stb_image():
+    """The reference's stb_image built from its source (oracle/_ref), or None
+    where it has not been built."""
+    global _stb
+    if _stb is None:
+        if not STB_PATH.exists():
+            return None
+        h = C.CDLL(str(STB_PATH))
+        h.stbi_load_from_memory.restype = C.POINTER(C.c_uint8)
+        h.stbi_load_from_memory.argtypes = [_U8, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int), C.c_int]
+        h.stbi_image_free.restype = None
+        h.stbi_image_free.argtypes = [C.c_void_p]
+        _stb = h
+    return _stb
+
+
+def stbi_load_rgba(data: bytes):
+    """stbi_load_from_memory(..., 4) of the reference's decoder: (h, w, 4)
+    uint8 array, or None if it rejects the file."""
+    h = stb_image()
+    buf = np.frombuffer(data, np.uint8).copy()
+    w, hh, n = C.c_int(), C.c_int(), C.c_int()
+    p = h.stbi_load_from_memory(buf.ctypes.data_as(_U8), len(data), C.byref(w), C.byref(hh), C.byref(n), 4)
+    if not p:
+        return None
+    out = np.ctypeslib.as_array(p, shape=(hh.value, w.value, 4)).copy()
+    h.stbi_image_free(p)
+    return out
 
 
 def lib():

@@ -12,12 +12,15 @@ from axiomr_amd.ingest import load_image, load_obj, write_png
 pytestmark = pytest.mark.gpu
 
 
-def write_assets(tmp_path):
+def write_assets(tmp_path, jpeg=False):
     v, f = S.uv_sphere(40, 48)
     tex = {"albedo.png": S.noise_texture(128, 128, 7), "nrm.png": S.normal_map(128, 128, 0.7, 8),
            "stripes.png": S.checker(64, 64, 4)}
     for name, img in tex.items():
         write_png(str(tmp_path / name), img)
+    if jpeg:   # the same albedo as a JPEG file (4:2:0, quality 85)
+        import PIL.Image
+        PIL.Image.fromarray(tex["albedo.png"][..., :3]).save(str(tmp_path / "albedo.jpg"), quality=85)
     lines = []
     for p in v:
         lines.append("v %.9g %.9g %.9g" % (p[0], p[1], p[2]))
@@ -33,7 +36,8 @@ def write_assets(tmp_path):
             lines.append("usemtl band")
         lines.append("f %d/%d/%d %d/%d/%d %d/%d/%d" % (a + 1, a + 1, a + 1, b + 1, b + 1, b + 1, c + 1, c + 1, c + 1))
     (tmp_path / "ball.obj").write_text("\n".join(lines) + "\n")
-    (tmp_path / "ball.mtl").write_text("newmtl skin\nNs 0.5\nmap_Kd albedo.png\nmap_Bump nrm.png\n\n"
+    albedo = "albedo.jpg" if jpeg else "albedo.png"
+    (tmp_path / "ball.mtl").write_text(f"newmtl skin\nNs 0.5\nmap_Kd {albedo}\nmap_Bump nrm.png\n\n"
                                        "newmtl band\nNs 0.2\nmap_Kd stripes.png\nbump nrm.png\n")
     return str(tmp_path / "ball.obj")
 
@@ -56,9 +60,12 @@ def scene_of(model, filt):
                        materials=mats, textures=names)
 
 
-@pytest.mark.parametrize("filt", [R.FILTER_NEAREST, R.FILTER_BILINEAR])
-def test_obj_model_matches_checker(ctx, tmp_path, filt):
-    path = write_assets(tmp_path)
+@pytest.mark.parametrize("filt,jpeg", [(R.FILTER_NEAREST, False), (R.FILTER_BILINEAR, False),
+                                       (R.FILTER_NEAREST, True)])
+def test_obj_model_matches_checker(ctx, tmp_path, filt, jpeg):
+    if jpeg:
+        pytest.importorskip("PIL.Image")
+    path = write_assets(tmp_path, jpeg)
     mesh = R.Mesh.from_obj(ctx, path, filter=filt)
     model = load_obj(path)
     assert len(model.groups) == 2 and [g.material for g in model.groups] == [1, 0]   # band, skin sorted
