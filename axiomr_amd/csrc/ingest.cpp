@@ -387,8 +387,8 @@ int axr_obj_counts(const axr_obj* o, uint64_t* nverts, uint64_t* nfaces, uint32_
 
 int axr_obj_arrays(const axr_obj* o, float* verts56, uint32_t* idx) {
     if (!o) return fail(AXR_ERR_INVALID, "null model");
-    if (verts56) std::memcpy(verts56, o->verts.data(), o->verts.size() * 4);
-    if (idx) std::memcpy(idx, o->idx.data(), o->idx.size() * 4);
+    if (verts56 && !o->verts.empty()) std::memcpy(verts56, o->verts.data(), o->verts.size() * 4);
+    if (idx && !o->idx.empty()) std::memcpy(idx, o->idx.data(), o->idx.size() * 4);
     return AXR_OK;
 }
 
