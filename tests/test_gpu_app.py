@@ -29,8 +29,9 @@ def write_model(tmp_path):
     lines.append("usemtl metal")
     lines += [f"f {a + 1}/{a + 1}/{a + 1} {b + 1}/{b + 1}/{b + 1} {c + 1}/{c + 1}/{c + 1}" for a, b, c in f]
     (tmp_path / "tank.obj").write_text("\n".join(lines) + "\n")
+    # the reference's PBR shader samples all five maps (shaders.hpp:284-399)
     (tmp_path / "tank.mtl").write_text("newmtl metal\nNs 0.4\nmap_Kd albedo.png\nmap_Bump nrm.png\n"
-                                       "map_Ns rough.png\nmap_Ks rough.png\n")
+                                       "map_Ns rough.png\nmap_Ks rough.png\nmap_A0 albedo.png\n")
     return str(tmp_path / "tank.obj")
 
 
