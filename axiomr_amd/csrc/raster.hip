@@ -1138,12 +1138,12 @@ __global__ __launch_bounds__(256) void k_big(KArgs ka, Arena ar) {
 // framebuffer (strict less, reference mergeTileResults), shade, store, and
 // leave the key empty for the next frame.
 
-// Returns whether the winner passed the depth test against dfb (the
-// framebuffer's depth, or the depth of a clear the draw absorbs) and was
-// written.
+// Shades the winner of key k at (px, py) if it passes the depth test
+// against dfb (the framebuffer's depth, or the depth of a clear the draw
+// absorbs): returns whether it did, with its depth and packed colour.
 template <class S>
-__device__ __forceinline__ bool resolve_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
-                                                  int py, size_t pix, float dfb) {
+__device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
+                                                int py, float dfb, float& z, uint32_t& bgra) {
     const uint32_t key = (uint32_t)k;
     const uint32_t s = key >> 3, j = key & 7u;
     const DevGroup G = group_of(V, s);
@@ -1180,7 +1180,7 @@ __device__ __forceinline__ bool resolve_vis_pixel(const ViewDesc& V, const Arena
     }
     RasterTri rt;
     raster_setup(r, rt);
-    float al, be, ga, z;
+    float al, be, ga;
     cover(rt, px, py, al, be, ga, z);
     if (!(z < dfb)) return false;   // reference mergeTileResults: strict less
     ShadeCtx c;
@@ -1198,10 +1198,21 @@ __device__ __forceinline__ bool resolve_vis_pixel(const ViewDesc& V, const Arena
             var[2] = S::vs(v2, V.u);
         }
     }
-    const vec4 col = S::fs(make3(al, be, ga), var, c);
+    bgra = pack_bgra(S::fs(make3(al, be, ga), var, c));
+    return true;
+}
+
+// Returns whether the winner passed the depth test against dfb and was
+// written.
+template <class S>
+__device__ __forceinline__ bool resolve_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
+                                                  int py, size_t pix, float dfb) {
+    float z;
+    uint32_t bgra;
+    if (!shade_vis_pixel<S>(V, ar, k, px, py, dfb, z, bgra)) return false;
     V.depth[pix] = z;
-    V.color[pix] = pack_bgra(col);
-    if (V.ids) V.ids[pix] = key;
+    V.color[pix] = bgra;
+    if (V.ids) V.ids[pix] = (uint32_t)k;
     return true;
 }
 
@@ -1305,6 +1316,87 @@ __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
 }
 
 // ---------------------------------------------------------------------------
+// The resolve of a plain draw that absorbs a clear, in two passes, so that
+// the shading need not wait for an earlier draw into the same framebuffer:
+// winners are tested against the clear depth, never against the planes.
+// k_shade shades every winner into its own key slot (depth bits << 32 |
+// packed colour; a winner behind the clear empties the slot); k_finish, the
+// only pass that touches the framebuffer, writes every pixel from its slot
+// or the clear and leaves the slots and bin flags empty.  A framebuffer
+// with an id plane takes the one-pass k_resolve (the slot has no room for
+// the winner's id).
+
+template <int T, int SK>
+__global__ __launch_bounds__(256) void k_shade(KArgs ka, Arena ar) {
+    using S = typename ShaderOf<SK>::type;
+    if (ar.counters[CNT_ERROR]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the per-frame counters of this mode were last needed by k_big
+        ar.counters[CNT_CLIP_FACES] = 0;
+        ar.counters[CNT_CLIP_RECS] = 0;
+        *big_counter(ar) = 0ull;
+    }
+    const ViewDesc& V = ka.one;
+    unsigned long long* vk = ar.vkeys + V.pix_base;
+    const uint32_t nw = gridDim.x * 4u;
+    for (uint32_t t = (blockIdx.x * 256u + threadIdx.x) >> 6; t < ka.total_tiles; t += nw) {
+        if (ar.tile_count[t] == 0u) continue;
+        const uint32_t local = t - V.tile_base;
+        const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
+        for (int p = (int)__lane_id(); p < T * T; p += 64) {
+            const int px = ox + (p % T), py = oy + (p / T);
+            if (px >= V.W || py >= V.H) continue;
+            const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
+            const unsigned long long k = vk[pix];
+            if (k == EMPTY_KEY) continue;
+            float z;
+            uint32_t bgra;
+            const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra);
+            vk[pix] = drawn ? ((unsigned long long)__float_as_uint(z) << 32) | bgra : EMPTY_KEY;
+        }
+    }
+}
+
+// One wave per strip of 64 pixels by one bin row (64 / T bins side by
+// side), one lane per column: each row of the strip is one 256-byte store
+// per plane.  A slot is read only in bins that had keys.
+template <int T>
+__global__ __launch_bounds__(256) void k_finish(KArgs ka, Arena ar) {
+    static_assert(64 % T == 0, "a strip holds whole bins");
+    const bool failed = ar.counters[CNT_ERROR] != 0u;
+    const ViewDesc& V = ka.one;
+    const ClearSpec cs{true, ka.clear_bgra, ka.clear_depth};
+    unsigned long long* vk = ar.vkeys + V.pix_base;
+    const uint32_t lane = __lane_id();
+    const uint32_t sx_n = ((uint32_t)V.W + 63u) / 64u;
+    const uint32_t nstrips = sx_n * (uint32_t)V.tiles_y;
+    const uint32_t nw = gridDim.x * 4u;
+    for (uint32_t st = (blockIdx.x * 256u + threadIdx.x) >> 6; st < nstrips; st += nw) {
+        const uint32_t sx = st % sx_n, by = st / sx_n;
+        const int px = (int)(sx * 64u + lane);
+        const bool in_x = px < V.W;
+        const uint32_t t = V.tile_base + by * (uint32_t)V.tiles_x + (uint32_t)px / (uint32_t)T;
+        const bool keys = !failed && in_x && ar.tile_count[t] != 0u;
+        for (int r = 0; r < T; ++r) {
+            const int py = (int)by * T + r;
+            if (!in_x || py >= V.H) continue;
+            const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
+            unsigned long long s = EMPTY_KEY;
+            if (keys) s = vk[pix];
+            if (s != EMPTY_KEY) {
+                vk[pix] = EMPTY_KEY;
+                V.color[pix] = (uint32_t)s;
+                V.depth[pix] = __uint_as_float((uint32_t)(s >> 32));
+            } else {
+                clear_pixel(V, pix, cs);
+            }
+        }
+        // every lane of a bin read its flag before the loop's loads returned
+        if (keys && (uint32_t)px % (uint32_t)T == 0u) ar.tile_count[t] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // framebuffer clear (reference framebuffer.cpp:26-42)
 
 // Planes start 256-byte aligned (hipMalloc, or 256-byte aligned offsets in
@@ -1366,10 +1458,27 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     if (ev) (void)hipEventRecord(ev[1], st);
     k_big<T, BATCH><<<1024, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[2], st);
+    const int rb = imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS));
+    if (!BATCH && wait_last && ka.clear && !ka.one.ids && ka.shader >= 0 && ka.shader < SHADER_COUNT) {
+        // shade at once, write the framebuffer after the earlier draws
+        switch (ka.shader) {
+        case SHADER_FLAT: k_shade<T, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_PHONG: k_shade<T, SHADER_PHONG><<<rb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_PBR: k_shade<T, SHADER_PBR><<<rb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_GOURAUD: k_shade<T, SHADER_GOURAUD><<<rb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_PHONG_POINT: k_shade<T, SHADER_PHONG_POINT><<<rb, 256, 0, st>>>(ka, ar); break;
+        default: k_shade<T, SHADER_FLAT_COLOR><<<rb, 256, 0, st>>>(ka, ar); break;
+        }
+        if (ev) (void)hipEventRecord(ev[3], st);
+        (void)hipStreamWaitEvent(st, wait_last, 0);
+        const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
+        k_finish<T><<<imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
+        if (ev) (void)hipEventRecord(ev[4], st);
+        return hipGetLastError();
+    }
     if (ev) (void)hipEventRecord(ev[3], st);
     // the framebuffer pass waits for what other streams were asked to do first
     if (wait_last) (void)hipStreamWaitEvent(st, wait_last, 0);
-    const int rb = imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS));
     switch (ka.shader) {
     case SHADER_FLAT: k_resolve<T, BATCH, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
     case SHADER_PHONG: k_resolve<T, BATCH, SHADER_PHONG><<<rb, 256, 0, st>>>(ka, ar); break;

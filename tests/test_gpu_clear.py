@@ -174,12 +174,14 @@ def quad_stack(n=20, width=1920, height=1080):
                        groups=[(0, 0, f.shape[0])], materials=[S.MaterialData()], face_colors=colors)
 
 
-@pytest.mark.parametrize("mode", [N.RASTER_BINNED, N.RASTER_VISIBILITY])
-def test_overflowing_frame_after_clear(mode):
+@pytest.mark.parametrize("mode,with_ids", [(N.RASTER_BINNED, True), (N.RASTER_VISIBILITY, True),
+                                           (N.RASTER_VISIBILITY, False)])
+def test_overflowing_frame_after_clear(mode, with_ids):
     """A fresh context's arena is too small for 40 clipped screen-filling
     triangles at 1080p: a synchronous draw grows it and redraws; an
     asynchronous one reports the overflow at sync and leaves the cleared
-    framebuffer, as if its clear had run on its own."""
+    framebuffer, as if its clear had run on its own (without an id plane the
+    asynchronous draw shades and writes in two passes)."""
     sc = quad_stack()
     p = H.params_of(sc)
     ref = oracle_sequence(sc, [("clear", TINT, float("inf")), ("draw", p)])
@@ -188,7 +190,7 @@ def test_overflowing_frame_after_clear(mode):
         try:
             ctx.set_config(tile_size=16, raster_threads=256, small_footprint=128, raster_mode=mode)
             gs = H.GpuScene(ctx, sc)
-            fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=True)
+            fb = R.Framebuffer(ctx, sc.width, sc.height, with_ids=with_ids)
             try:
                 fb.clear(TINT)
                 if asynchronous:
@@ -197,10 +199,12 @@ def test_overflowing_frame_after_clear(mode):
                     assert ctx._lib.axr_sync(ctx.handle) == -4          # AXR_ERR_OVERFLOW
                     bgra, depth, ids = fb.read()
                     assert (bgra.view(np.uint32) == TINT.packed_bgra()).all() and np.isinf(depth).all()
-                    assert (ids == 0xFFFFFFFF).all()
+                    assert ids is None if not with_ids else (ids == 0xFFFFFFFF).all()
                     fb.clear(TINT)
                 draw(ctx, fb, gs, p)                                    # grows (again) if needed, redraws
-                H.assert_parity(H.compare(fb.read(), ref), f"quad stack mode {mode} async {asynchronous}")
+                got = fb.read()
+                H.assert_parity(H.compare(got, ref if with_ids else (ref[0], ref[1], None)),
+                                f"quad stack mode {mode} async {asynchronous}")
             finally:
                 fb.close()
                 gs.close()
@@ -247,3 +251,42 @@ def test_async_draws_keep_call_order(ctx):
         gs.close()
     H.assert_parity(H.compare(got_a, oracle_sequence(sc, steps_a)), "six async draws into one framebuffer")
     H.assert_parity(H.compare(got_b, oracle_sequence(sc, steps_b)), "interleaved framebuffer")
+
+
+def test_async_frames_without_ids_shade_ahead(ctx):
+    """Asynchronous clear-then-draw frames into framebuffers without an id
+    plane shade before the earlier frames have written and write after
+    them: frames round-robin over three framebuffers, a clear at the median
+    depth (winners behind it are dropped), a second draw without a clear
+    between (the one-pass resolve, depth-tested against what the two-pass
+    one wrote), and other shader kinds."""
+    sc = small_scene()
+    d = median_depth(sc)
+    light = (0.2, -1.0, -0.3)
+    variants = {4: sc,
+                0: S.with_shader(sc, {"kind": 0, "lightDirection": light}),
+                1: S.with_shader(sc, {"kind": 1, "lightDirection": light, "lightColor": (0.6, 0.6, 0.6)}),
+                3: S.with_shader(sc, {"kind": 3, "lightDirection": light})}
+    gs = H.GpuScene(ctx, sc)
+    fbs = [R.Framebuffer(ctx, sc.width, sc.height) for _ in range(3)]
+    steps = [[] for _ in fbs]
+    try:
+        for i, kind in enumerate([4, 0, 1, 3, 4, 4, 1, 4, 0]):
+            k, v = i % 3, variants[kind]
+            col, dep = R.Color(20 * i, 7, 99, 255), (d if i == 4 else float("inf"))
+            fbs[k].clear(col, dep)
+            steps[k] = [("clear", col, dep)]
+            draw(ctx, fbs[k], gs, rotated(v, 11.0 * i, N.AXR_DRAW_ASYNC))
+            steps[k].append(("draw", rotated(v, 11.0 * i)))
+            if i == 6:
+                draw(ctx, fbs[k], gs, rotated(v, -40.0, N.AXR_DRAW_ASYNC))
+                steps[k].append(("draw", rotated(v, -40.0)))
+        ctx.sync()
+        got = [fb.read() for fb in fbs]
+    finally:
+        for fb in fbs:
+            fb.close()
+        gs.close()
+    for k in range(3):
+        ref = oracle_sequence(sc, steps[k])
+        H.assert_parity(H.compare(got[k], (ref[0], ref[1], None)), f"framebuffer {k}")
