@@ -201,6 +201,9 @@ struct KArgs {
     int32_t clear = 0;          // visibility mode: k_resolve also clears every view's
     uint32_t clear_bgra = 0;    // framebuffer to these values (a clear the draw
     float clear_depth = 0.0f;   // absorbs instead of a pass of its own)
+    int32_t two_pass = 0;       // plain visibility-mode draw with a clear and no id
+                                // plane: shade (k_shade) apart from the framebuffer
+                                // writes (k_finish), which alone wait for earlier draws
     ViewDesc one;
 };
 

@@ -788,6 +788,10 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
         ka.clear_depth = fb->clear_depth;
     }
     const bool async = (p->flags & AXR_DRAW_ASYNC) != 0;
+    // an asynchronous draw that absorbs a clear into a framebuffer without
+    // ids shades apart from its framebuffer writes (also when profiled, so
+    // the stage times are those of the same kernels)
+    ka.two_pass = async && ka.clear && !fb->ids;
     if (async && ka.mode == RASTER_VISIBILITY && !ctx->profiling) return draw_pipelined(ctx, fb, ka, fs);
     int r;
     if ((fb->clear_pending && !ka.clear && (r = fb_settle(fb))) || (r = ctx_join(ctx)) ||

@@ -1377,18 +1377,88 @@ __global__ __launch_bounds__(256) void k_finish(KArgs ka, Arena ar) {
         const bool in_x = px < V.W;
         const uint32_t t = V.tile_base + by * (uint32_t)V.tiles_x + (uint32_t)px / (uint32_t)T;
         const bool keys = !failed && in_x && ar.tile_count[t] != 0u;
+        const int rows = imin(T, V.H - (int)by * T);
+        const size_t pix0 = (size_t)by * T * (size_t)V.W + (size_t)px;
+        // all slot loads first: the stores below may alias them for the
+        // compiler, which would otherwise wait for each row's load in turn
+        unsigned long long sl[T];
+#pragma unroll
+        for (int r = 0; r < T; ++r) sl[r] = keys && r < rows ? vk[pix0 + (size_t)r * V.W] : EMPTY_KEY;
+        if (!in_x) continue;
+#pragma unroll
         for (int r = 0; r < T; ++r) {
-            const int py = (int)by * T + r;
-            if (!in_x || py >= V.H) continue;
-            const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
-            unsigned long long s = EMPTY_KEY;
-            if (keys) s = vk[pix];
-            if (s != EMPTY_KEY) {
+            if (r >= rows) break;
+            const size_t pix = pix0 + (size_t)r * V.W;
+            if (sl[r] != EMPTY_KEY) {
                 vk[pix] = EMPTY_KEY;
-                V.color[pix] = (uint32_t)s;
-                V.depth[pix] = __uint_as_float((uint32_t)(s >> 32));
+                V.color[pix] = (uint32_t)sl[r];
+                V.depth[pix] = __uint_as_float((uint32_t)(sl[r] >> 32));
             } else {
                 clear_pixel(V, pix, cs);
+            }
+        }
+        // every lane of a bin read its flag before the loop's loads returned
+        if (keys && (uint32_t)px % (uint32_t)T == 0u) ar.tile_count[t] = 0u;
+    }
+}
+
+// k_finish for a width that is a multiple of 4 (every row then starts
+// 16-byte aligned in the 256-byte aligned planes): four pixels per lane,
+// strips of 256 pixels by one bin row, 16-byte loads and stores.
+template <int T>
+__global__ __launch_bounds__(256) void k_finish4(KArgs ka, Arena ar) {
+    static_assert(256 % T == 0 && T % 4 == 0, "a strip holds whole bins, a lane's pixels one bin");
+    constexpr int RB = T < 8 ? T : 8;   // rows whose slots are loaded together
+    const bool failed = ar.counters[CNT_ERROR] != 0u;
+    const ViewDesc& V = ka.one;
+    unsigned long long* vk = ar.vkeys + V.pix_base;
+    const uint4 cv = make_uint4(ka.clear_bgra, ka.clear_bgra, ka.clear_bgra, ka.clear_bgra);
+    const float4 dv = make_float4(ka.clear_depth, ka.clear_depth, ka.clear_depth, ka.clear_depth);
+    const ulonglong2 ev2 = make_ulonglong2(EMPTY_KEY, EMPTY_KEY);
+    const uint32_t lane = __lane_id();
+    const uint32_t sx_n = ((uint32_t)V.W + 255u) / 256u;
+    const uint32_t nstrips = sx_n * (uint32_t)V.tiles_y;
+    const uint32_t nw = gridDim.x * 4u;
+    for (uint32_t st = (blockIdx.x * 256u + threadIdx.x) >> 6; st < nstrips; st += nw) {
+        const uint32_t sx = st % sx_n, by = st / sx_n;
+        const int px = (int)(sx * 256u + 4u * lane);
+        if (px >= V.W) continue;   // W % 4 == 0: the lane's four pixels are all inside or all outside
+        const uint32_t t = V.tile_base + by * (uint32_t)V.tiles_x + (uint32_t)px / (uint32_t)T;
+        const bool keys = !failed && ar.tile_count[t] != 0u;
+        const int rows = imin(T, V.H - (int)by * T);
+        const size_t pix0 = (size_t)by * T * (size_t)V.W + (size_t)px;
+        for (int r0 = 0; r0 < rows; r0 += RB) {
+            // all slot loads of RB rows first: the stores below may alias
+            // them for the compiler, which would otherwise wait for each
+            ulonglong2 a[RB], b[RB];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                a[r] = b[r] = ev2;
+                if (keys && r0 + r < rows) {
+                    const ulonglong2* q = reinterpret_cast<const ulonglong2*>(vk + pix0 + (size_t)(r0 + r) * V.W);
+                    a[r] = q[0];
+                    b[r] = q[1];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                if (r0 + r >= rows) break;
+                const size_t pix = pix0 + (size_t)(r0 + r) * V.W;
+                const unsigned long long s0 = a[r].x, s1 = a[r].y, s2 = b[r].x, s3 = b[r].y;
+                const bool any = (s0 & s1 & s2 & s3) != EMPTY_KEY;
+                uint4 c = cv;
+                float4 d = dv;
+                if (any) {
+                    if (s0 != EMPTY_KEY) { c.x = (uint32_t)s0; d.x = __uint_as_float((uint32_t)(s0 >> 32)); }
+                    if (s1 != EMPTY_KEY) { c.y = (uint32_t)s1; d.y = __uint_as_float((uint32_t)(s1 >> 32)); }
+                    if (s2 != EMPTY_KEY) { c.z = (uint32_t)s2; d.z = __uint_as_float((uint32_t)(s2 >> 32)); }
+                    if (s3 != EMPTY_KEY) { c.w = (uint32_t)s3; d.w = __uint_as_float((uint32_t)(s3 >> 32)); }
+                    ulonglong2* q = reinterpret_cast<ulonglong2*>(vk + pix);
+                    q[0] = ev2;
+                    q[1] = ev2;
+                }
+                *reinterpret_cast<uint4*>(V.color + pix) = c;
+                *reinterpret_cast<float4*>(V.depth + pix) = d;
             }
         }
         // every lane of a bin read its flag before the loop's loads returned
@@ -1447,8 +1517,9 @@ struct LaunchShape {
 // counts and per-frame counters zero) and leaves it clean.
 constexpr int RESOLVE_MAX_BLOCKS = 16384;
 
-// Visibility mode: k_vis, k_clip, k_big, k_resolve.  Events: after k_vis,
-// k_clip, k_big, (nothing), k_resolve.
+// Visibility mode: k_vis, k_clip, k_big, k_resolve (or k_shade, k_finish
+// for a two-pass draw).  Events: after k_vis, k_clip, k_big, (nothing or
+// k_shade), k_resolve (or k_finish).
 template <int T, bool BATCH>
 static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t wait_last) {
     const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
@@ -1459,7 +1530,7 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     k_big<T, BATCH><<<1024, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[2], st);
     const int rb = imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS));
-    if (!BATCH && wait_last && ka.clear && !ka.one.ids && ka.shader >= 0 && ka.shader < SHADER_COUNT) {
+    if (!BATCH && ka.two_pass && ka.clear && !ka.one.ids && ka.shader >= 0 && ka.shader < SHADER_COUNT) {
         // shade at once, write the framebuffer after the earlier draws
         switch (ka.shader) {
         case SHADER_FLAT: k_shade<T, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
@@ -1470,9 +1541,14 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
         default: k_shade<T, SHADER_FLAT_COLOR><<<rb, 256, 0, st>>>(ka, ar); break;
         }
         if (ev) (void)hipEventRecord(ev[3], st);
-        (void)hipStreamWaitEvent(st, wait_last, 0);
-        const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
-        k_finish<T><<<imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
+        if (wait_last) (void)hipStreamWaitEvent(st, wait_last, 0);
+        if (ka.one.W % 4 == 0) {
+            const uint32_t strips = (((uint32_t)ka.one.W + 255u) / 256u) * (uint32_t)ka.one.tiles_y;
+            k_finish4<T><<<imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
+        } else {
+            const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
+            k_finish<T><<<imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
+        }
         if (ev) (void)hipEventRecord(ev[4], st);
         return hipGetLastError();
     }

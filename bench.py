@@ -82,10 +82,12 @@ def apply_config(ctx, workload, tile, use_tuned):
     return ctx.config()
 
 
-def stage_names(cfg):
-    """Names of the five event-timed stages in the context's raster mode."""
+def stage_names(cfg, two_pass=False):
+    """Names of the five event-timed stages in the context's raster mode;
+    two_pass: visibility-mode draws that resolve in two launches (an
+    asynchronous draw with a clear into a framebuffer without ids)."""
     if cfg.get("raster_mode", 0) == 1:
-        return ("vis", "clip", "big", "none", "resolve")
+        return ("vis", "clip", "big", "shade", "finish") if two_pass else ("vis", "clip", "big", "none", "resolve")
     return ("setup", "clip", "scan", "scatter", "raster")
 
 
@@ -132,7 +134,7 @@ def run_plain(scene, steps, warmup, tile, use_tuned):
     ctx.sync()
     st = ctx.stats()
     ctx.set_profiling(False)
-    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(stage_names(cfg))}
+    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(stage_names(cfg, True))}
     frame_ms = st.frame_ms_total / max(1, st.frames)
     in_use, arena = ctx.memory()
     fb.close()
@@ -383,10 +385,13 @@ def measured_traffic(kname, tunables):
     data = json.loads(TRAFFIC_FILE.read_text())
     if data.get("tunables") != tunables:
         return None
-    hits = [v for k, v in data.get("kernels", {}).items() if k.startswith(kname + "<") or k == kname]
-    if len(hits) != 1:
-        return None
-    return int(round(hits[0]["traffic_bytes"]))
+    total = 0.0
+    for part in kname.split("+"):   # "k_shade+k_finish": the sum of both launches
+        hits = [v for k, v in data.get("kernels", {}).items() if k.startswith(part + "<") or k == part]
+        if len(hits) != 1:
+            return None
+        total += hits[0]["traffic_bytes"]
+    return int(round(total))
 
 
 def copy_calibration(mib, reps=5):
@@ -450,13 +455,20 @@ def main():
     ntex = 2
     log(f"[bench] plain path: {fps:.1f} frames/s ({ms:.3f} ms/frame), stages {r['stages_ms']}")
 
-    stages = r["stages_ms"]
+    stages = dict(r["stages_ms"])
+    if "shade" in stages:
+        # the resolve runs as two launches, k_shade then k_finish: priced
+        # together, as the one k_resolve they replace
+        stages["resolve"] = stages["shade"] + stages["finish"]
+        for k in ("shade", "finish"):
+            stages.pop(k)
     dominant = max(stages, key=stages.get)
     if dominant in ("setup", "vis"):
         kbytes, kname = setup_bytes(nverts, nfaces), "k_" + dominant
     elif "resolve" in stages:
         dominant = "resolve"
-        kbytes, kname = resolve_bytes(W, H, r["covered"], ntex), "k_resolve"
+        kbytes = resolve_bytes(W, H, r["covered"], ntex)
+        kname = "k_shade+k_finish" if "shade" in r["stages_ms"] else "k_resolve"
     else:
         dominant = "raster"
         kbytes, kname = raster_bytes(r["covered"], ntex), "k_raster"

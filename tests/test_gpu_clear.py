@@ -253,14 +253,25 @@ def test_async_draws_keep_call_order(ctx):
     H.assert_parity(H.compare(got_b, oracle_sequence(sc, steps_b)), "interleaved framebuffer")
 
 
-def test_async_frames_without_ids_shade_ahead(ctx):
+@pytest.mark.parametrize("width,height,tile", [(200, 150, 8), (333, 197, 8), (333, 197, 16), (256, 130, 32),
+                                               (250, 130, 32)])
+def test_async_frames_without_ids_shade_ahead(width, height, tile):
     """Asynchronous clear-then-draw frames into framebuffers without an id
     plane shade before the earlier frames have written and write after
     them: frames round-robin over three framebuffers, a clear at the median
     depth (winners behind it are dropped), a second draw without a clear
     between (the one-pass resolve, depth-tested against what the two-pass
-    one wrote), and other shader kinds."""
-    sc = small_scene()
+    one wrote), and other shader kinds.  Widths that are and are not a
+    multiple of 4 (16-byte and 4-byte framebuffer writes) and partial bins."""
+    ctx = R.Context(0)
+    try:
+        ctx.set_config(tile_size=tile, raster_mode=N.RASTER_VISIBILITY)
+        _frames_without_ids(ctx, S.icosphere_scene(n=16, width=width, height=height, tex=64))
+    finally:
+        ctx.close()
+
+
+def _frames_without_ids(ctx, sc):
     d = median_depth(sc)
     light = (0.2, -1.0, -0.3)
     variants = {4: sc,
