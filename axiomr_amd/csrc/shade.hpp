@@ -36,6 +36,36 @@ AXR_HD vec4 texel(const DevTexture& t, int x, int y) {
                 (float)((p >> 16) & 255u) * inv255, (float)(p >> 24) * inv255};
 }
 
+// A texel's four bytes as floats 0..255 (the bilinear filter blends these
+// and scales by 1/255 once).
+AXR_HD vec4 texel_bytes(const DevTexture& t, int x, int y) {
+    uint32_t p = t.texels[(size_t)y * (size_t)t.w + (size_t)x];
+    return vec4{(float)(p & 255u), (float)((p >> 8) & 255u), (float)((p >> 16) & 255u), (float)(p >> 24)};
+}
+
+// Colour-only arithmetic with fused multiply-adds.  A fused multiply-add
+// differs from the separately rounded pair by at most one rounding of the
+// product (~6e-8 relative); through the shading terms that stays orders of
+// magnitude below one 8-bit step (1/255), which is the tolerance colour is
+// held to.  Barycentrics, texture coordinates, coverage and depth never go
+// through these.
+AXR_HD float cdot(vec3 a, vec3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+AXR_HD vec3 cmad(vec3 a, float s, vec3 c) { return make3(fmaf(a.x, s, c.x), fmaf(a.y, s, c.y), fmaf(a.z, s, c.z)); }
+AXR_HD vec4 cmad4(vec4 a, float s, vec4 c) {
+    return make4(fmaf(a.x, s, c.x), fmaf(a.y, s, c.y), fmaf(a.z, s, c.z), fmaf(a.w, s, c.w));
+}
+AXR_HD vec3 cbary3(vec3 b, vec3 v0, vec3 v1, vec3 v2) { return cmad(v2, b.z, cmad(v1, b.y, v0 * b.x)); }
+AXR_HD float cbary1(vec3 b, float v0, float v1, float v2) { return fmaf(b.z, v2, fmaf(b.y, v1, b.x * v0)); }
+AXR_HD vec3 cmul(const mat3& m, vec3 v) { return cmad(m.c[2], v.z, cmad(m.c[1], v.y, m.c[0] * v.x)); }
+AXR_HD vec3 cdir(const mat4& m, float x, float y, float z) {
+    return cmad(xyz(m.c[2]), z, cmad(xyz(m.c[1]), y, xyz(m.c[0]) * x));
+}
+AXR_HD vec3 cpoint(const mat4& m, float x, float y, float z) {
+    return cmad(xyz(m.c[2]), z, cmad(xyz(m.c[1]), y, cmad(xyz(m.c[0]), x, xyz(m.c[3]))));
+}
+AXR_HD vec3 creflect(vec3 i, vec3 n) { return cmad(n, -2.0f * cdot(n, i), i); }
+AXR_HD vec4 cmix(vec4 x, vec4 y, float a) { return cmad4(y - x, a, x); }
+
 AXR_HD vec4 sample_nearest(const DevTexture& t, vec2 uv) {
     int x = cvt_i32(uv.x * (float)(t.w - 1));
     int y = cvt_i32(uv.y * (float)(t.h - 1));
@@ -55,11 +85,11 @@ AXR_HD vec4 sample_bilinear(const DevTexture& t, vec2 uv) {
     x0 = imax(0, imin(x0, t.w - 1)); x1 = imax(0, imin(x1, t.w - 1));
     y0 = imax(0, imin(y0, t.h - 1)); y1 = imax(0, imin(y1, t.h - 1));
     int r0 = t.h - 1 - y0, r1 = t.h - 1 - y1;
-    vec4 t00 = texel(t, x0, r0), t10 = texel(t, x1, r0);
-    vec4 t01 = texel(t, x0, r1), t11 = texel(t, x1, r1);
-    vec4 top = mix(t00, t10, fx);
-    vec4 bot = mix(t01, t11, fx);
-    return mix(top, bot, fy);
+    vec4 t00 = texel_bytes(t, x0, r0), t10 = texel_bytes(t, x1, r0);
+    vec4 t01 = texel_bytes(t, x0, r1), t11 = texel_bytes(t, x1, r1);
+    vec4 top = cmix(t00, t10, fx);
+    vec4 bot = cmix(t01, t11, fx);
+    return cmix(top, bot, fy) * (1.0f / 255.0f);
 }
 
 AXR_HD vec4 sample(const DevTexture& t, vec2 uv) {
@@ -76,7 +106,7 @@ AXR_HD vec4 sample(const DevTexture& t, vec2 uv) {
 // discontinuous in uv), and so does everything that decides coverage, depth
 // and winners.
 #if defined(__HIP_DEVICE_COMPILE__)
-AXR_HD vec3 fnormalize(vec3 v) { return v * __builtin_amdgcn_rsqf(dot(v, v)); }
+AXR_HD vec3 fnormalize(vec3 v) { return v * __builtin_amdgcn_rsqf(cdot(v, v)); }
 AXR_HD float fpow(float x, float e) {
     // x >= 0 at every call site; x^0 = 1 (also for x = 0, as powf)
     return e == 0.0f ? 1.0f : __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
@@ -93,20 +123,16 @@ struct ShadeCtx {
     uint32_t prim;     // mesh face index of the primitive
 };
 
-AXR_HD vec3 bary3(vec3 b, vec3 v0, vec3 v1, vec3 v2) { return (b.x * v0 + b.y * v1) + b.z * v2; }
 AXR_HD vec2 bary2(vec3 b, vec2 v0, vec2 v1, vec2 v2) { return (b.x * v0 + b.y * v1) + b.z * v2; }
-AXR_HD float bary1(vec3 b, float v0, float v1, float v2) { return (b.x * v0 + b.y * v1) + b.z * v2; }
-
-AXR_HD vec3 model_dir(const mat4& m, float x, float y, float z) { return xyz(m * make4(x, y, z, 0.0f)); }
 
 struct FlatShader {
     struct Var { vec3 n; };
     static AXR_HD Var vs(const Vertex56& v, const Uniforms& u) {
-        return Var{u.normal_mat * make3(v.nx, v.ny, v.nz)};
+        return Var{cmul(u.normal_mat, make3(v.nx, v.ny, v.nz))};
     }
     static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
-        vec3 n = fnormalize(bary3(bar, t[0].n, t[1].n, t[2].n));
-        float i = sclamp(dot(-c.u->light_dir, n), 0.0f, 1.0f);
+        vec3 n = fnormalize(cbary3(bar, t[0].n, t[1].n, t[2].n));
+        float i = sclamp(cdot(-c.u->light_dir, n), 0.0f, 1.0f);
         return vec4{1.0f * i, 1.0f * i, 1.0f * i, 1.0f * i};
     }
 };
@@ -116,32 +142,32 @@ struct TbnVar { vec2 uv; vec3 wp; vec3 t, b, n; };
 AXR_HD TbnVar tbn_vs(const Vertex56& v, const Uniforms& u) {
     TbnVar o;
     o.uv = make2(v.u, v.v);
-    o.wp = xyz(u.model * make4(v.px, v.py, v.pz, 1.0f));
-    o.t = fnormalize(model_dir(u.model, v.tx, v.ty, v.tz));
-    o.b = fnormalize(model_dir(u.model, v.bx, v.by, v.bz));
-    o.n = fnormalize(model_dir(u.model, v.nx, v.ny, v.nz));
+    o.wp = cpoint(u.model, v.px, v.py, v.pz);
+    o.t = fnormalize(cdir(u.model, v.tx, v.ty, v.tz));
+    o.b = fnormalize(cdir(u.model, v.bx, v.by, v.bz));
+    o.n = fnormalize(cdir(u.model, v.nx, v.ny, v.nz));
     return o;
 }
 // Interpolated TBN columns: (b.x*M0 + b.y*M1) + b.z*M2, column by column.
 AXR_HD void tbn_interp(vec3 bar, const TbnVar* t, vec3& iT, vec3& iB, vec3& iN) {
-    iT = (bar.x * t[0].t + bar.y * t[1].t) + bar.z * t[2].t;
-    iB = (bar.x * t[0].b + bar.y * t[1].b) + bar.z * t[2].b;
-    iN = (bar.x * t[0].n + bar.y * t[1].n) + bar.z * t[2].n;
+    iT = cbary3(bar, t[0].t, t[1].t, t[2].t);
+    iB = cbary3(bar, t[0].b, t[1].b, t[2].b);
+    iN = cbary3(bar, t[0].n, t[1].n, t[2].n);
 }
 // Normal-mapped world normal, albedo and position (reference shaders.hpp:172-218).
 AXR_HD void phong_surface(vec3 bar, const TbnVar* t, const DevMaterial& m, vec3& normal, vec4& albedo, vec3& fragPos) {
     vec2 uv = bary2(bar, t[0].uv, t[1].uv, t[2].uv);
     vec4 nmv = sample(m.bump, uv);
-    vec3 nms = fnormalize(xyz(nmv) * 2.0f - make3(1.0f, 1.0f, 1.0f));
+    vec3 nms = fnormalize(cmad(xyz(nmv), 2.0f, make3(-1.0f, -1.0f, -1.0f)));
     vec3 iT, iB, iN;
     tbn_interp(bar, t, iT, iB, iN);
     vec3 N = fnormalize(iN);
-    vec3 T = fnormalize(iT - N * dot(N, iT));
+    vec3 T = fnormalize(cmad(N, -cdot(N, iT), iT));
     vec3 B = cross(N, T);
     mat3 f{{T, B, N}};
     albedo = sample(m.diffuse, uv);
-    normal = fnormalize(f * nms);
-    fragPos = bary3(bar, t[0].wp, t[1].wp, t[2].wp);
+    normal = fnormalize(cmul(f, nms));
+    fragPos = cbary3(bar, t[0].wp, t[1].wp, t[2].wp);
 }
 
 struct PhongShader {
@@ -153,13 +179,11 @@ struct PhongShader {
         const Uniforms& u = *c.u;
         vec3 viewDir = fnormalize(u.camera_pos - fragPos);
         vec3 lightDir = -u.light_dir;
-        vec3 ambient = 0.1f * u.light_color;
-        float diff = smax(dot(normal, lightDir), 0.0f);
-        vec3 diffuse = diff * u.light_color;
-        vec3 reflectDir = reflect(-lightDir, normal);
-        float spec = fpow(smax(dot(viewDir, reflectDir), 0.0f), c.m->specular_exponent * 50.0f);
-        vec3 specular = (0.5f * spec) * u.light_color;
-        return make4(((ambient + diffuse) + specular) * xyz(albedo), 1.0f);
+        float diff = smax(cdot(normal, lightDir), 0.0f);
+        vec3 reflectDir = creflect(-lightDir, normal);
+        float spec = fpow(smax(cdot(viewDir, reflectDir), 0.0f), c.m->specular_exponent * 50.0f);
+        // ambient + diffuse + specular = light_color * (0.1 + diff + 0.5 spec)
+        return make4((u.light_color * fmaf(0.5f, spec, 0.1f + diff)) * xyz(albedo), 1.0f);
     }
 };
 
@@ -175,13 +199,11 @@ struct PhongPointShader {
         vec3 sum = make3(0.0f, 0.0f, 0.0f);
         for (int l = 0; l < u.num_lights; ++l) {
             vec3 L = fnormalize(u.light_pos[l] - fragPos);
-            vec3 ambient = 0.1f * u.light_colors[l];
-            float diff = smax(dot(normal, L), 0.0f);
-            vec3 diffuse = diff * u.light_colors[l];
-            vec3 reflectDir = reflect(-L, normal);
-            float spec = fpow(smax(dot(viewDir, reflectDir), 0.0f), e);
-            vec3 specular = (0.5f * spec) * u.light_colors[l];
-            sum = sum + ((ambient + diffuse) + specular);
+            float diff = smax(cdot(normal, L), 0.0f);
+            vec3 reflectDir = creflect(-L, normal);
+            float spec = fpow(smax(cdot(viewDir, reflectDir), 0.0f), e);
+            // ambient + diffuse + specular = colour * (0.1 + diff + 0.5 spec)
+            sum = cmad(u.light_colors[l], fmaf(0.5f, spec, 0.1f + diff), sum);
         }
         return make4(sum * xyz(albedo), 1.0f);
     }
@@ -198,7 +220,7 @@ struct PBRShader {
         const Uniforms& u = *c.u;
         vec2 uv = bary2(bar, t[0].uv, t[1].uv, t[2].uv);
         vec4 nmv = sample(m.bump, uv);
-        vec3 nms = fnormalize((xyz(nmv) * 2.0f) - 1.0f);
+        vec3 nms = fnormalize(cmad(xyz(nmv), 2.0f, make3(-1.0f, -1.0f, -1.0f)));
         vec3 iT, iB, iN;
         tbn_interp(bar, t, iT, iB, iN);
         mat3 itbn{{iT, iB, iN}};
@@ -211,14 +233,14 @@ struct PBRShader {
         float omm = 1.0f - metallic;
         // The reference re-orthogonalises the frame but lights with the
         // interpolated one (shaders.hpp:316-343); only the latter matters.
-        vec3 normal = fnormalize(itbn * nms);
-        vec3 fragPos = bary3(bar, t[0].wp, t[1].wp, t[2].wp);
+        vec3 normal = fnormalize(cmul(itbn, nms));
+        vec3 fragPos = cbary3(bar, t[0].wp, t[1].wp, t[2].wp);
         vec3 viewDir = fnormalize(u.camera_pos - fragPos);
         vec3 lightDir = -u.light_dir;
         vec3 halfway = fnormalize(lightDir + viewDir);
-        float NdotH = smax(dot(normal, halfway), 0.0f);
-        float NdotV = smax(dot(normal, viewDir), 0.0f);
-        float NdotL = smax(dot(normal, lightDir), 0.0f);
+        float NdotH = smax(cdot(normal, halfway), 0.0f);
+        float NdotV = smax(cdot(normal, viewDir), 0.0f);
+        float NdotL = smax(cdot(normal, lightDir), 0.0f);
         vec3 c04 = make3(0.04f, 0.04f, 0.04f);
         vec3 F0 = c04 + metallic * (albedo - c04);
         float NdotH2 = NdotH * NdotH;
@@ -229,7 +251,7 @@ struct PBRShader {
         float nvk = NdotV * (1.0f - k) + k;
         float nlk = NdotL * (1.0f - k) + k;
         float G = (NdotV / nvk) * (NdotL / nlk);
-        float cosT = smax(dot(halfway, normal), 0.0f);
+        float cosT = smax(cdot(halfway, normal), 0.0f);
         float omc = 1.0f - cosT;
         float term = omc * omc; term = term * term; term = term * omc;
         vec3 one = make3(1.0f, 1.0f, 1.0f);
@@ -250,11 +272,11 @@ struct PBRShader {
 struct GouraudShader {
     struct Var { float i; vec2 uv; };
     static AXR_HD Var vs(const Vertex56& v, const Uniforms& u) {
-        vec3 N = fnormalize(model_dir(u.model, v.nx, v.ny, v.nz));
-        return Var{sclamp(dot(-u.light_dir, N), 0.0f, 1.0f), make2(v.u, v.v)};
+        vec3 N = fnormalize(cdir(u.model, v.nx, v.ny, v.nz));
+        return Var{sclamp(cdot(-u.light_dir, N), 0.0f, 1.0f), make2(v.u, v.v)};
     }
     static AXR_HD vec4 fs(vec3 bar, const Var* t, const ShadeCtx& c) {
-        float i = bary1(bar, t[0].i, t[1].i, t[2].i);
+        float i = cbary1(bar, t[0].i, t[1].i, t[2].i);
         vec2 uv = bary2(bar, t[0].uv, t[1].uv, t[2].uv);
         vec4 albedo = sample(c.m->diffuse, uv);
         return make4(xyz(albedo) * (0.1f + i), 1.0f);
