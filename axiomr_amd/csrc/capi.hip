@@ -285,16 +285,24 @@ int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const Frame
                 c[CNT_NEED_BIG_ITEMS]);
 }
 
+static FrameArena& arena_no(axr_ctx* ctx, int k) { return k ? ctx->alt[k - 1] : ctx->main; }
+
 int ctx_join(axr_ctx* ctx) {
-    if (!ctx->alt.stream) return AXR_OK;
-    hipError_t e = hipEventRecord(ctx->join_ev[1], ctx->alt.stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->main.stream, ctx->join_ev[1], 0);
-    return e == hipSuccess ? AXR_OK : hip_fail(ctx->err, e, "stream order");
+    hipError_t e = hipSuccess;
+    for (int k = 1; k < axr_ctx::kArenas && e == hipSuccess; ++k) {
+        if (!ctx->alt[k - 1].stream) continue;
+        e = hipEventRecord(ctx->join_ev[k], ctx->alt[k - 1].stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->main.stream, ctx->join_ev[k], 0);
+    }
+    if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
+    ctx->last_arena = 0;
+    return AXR_OK;
 }
 
 int ctx_sync_all(axr_ctx* ctx) {
     hipError_t e = hipStreamSynchronize(ctx->main.stream);
-    if (e == hipSuccess && ctx->alt.stream) e = hipStreamSynchronize(ctx->alt.stream);
+    for (int k = 1; k < axr_ctx::kArenas && e == hipSuccess; ++k)
+        if (ctx->alt[k - 1].stream) e = hipStreamSynchronize(ctx->alt[k - 1].stream);
     return e == hipSuccess ? AXR_OK : hip_fail(ctx->err, e, "hipStreamSynchronize");
 }
 
@@ -432,7 +440,7 @@ int axr_ctx_create(int device, uint64_t hbm_budget_bytes, axr_ctx** out) {
 int axr_ctx_destroy(axr_ctx* ctx) {
     if (!ctx) return AXR_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
-    arena_destroy(ctx, ctx->alt);
+    for (auto& a : ctx->alt) arena_destroy(ctx, a);
     arena_destroy(ctx, ctx->main);
     for (auto& e : ctx->join_ev)
         if (e) (void)hipEventDestroy(e);
@@ -518,8 +526,11 @@ int axr_sync(axr_ctx* ctx) {
     int r = ctx_sync_all(ctx);
     if (r) return r;
     r = arena_settle_overflow(ctx, ctx->main);
-    const int r2 = ctx->alt.stream ? arena_settle_overflow(ctx, ctx->alt) : AXR_OK;
-    return r ? r : r2;
+    for (auto& a : ctx->alt) {
+        const int r2 = a.stream ? arena_settle_overflow(ctx, a) : AXR_OK;
+        if (!r) r = r2;
+    }
+    return r;
 }
 
 // ---- resources -------------------------------------------------------------
@@ -736,26 +747,27 @@ int axr_fb_write_png(axr_fb* fb, const char* path) {
 // ---- drawMesh ----------------------------------------------------------------
 
 // An asynchronous visibility-mode draw on the arena whose turn it is: its
-// setup passes go on that arena's stream at once; its resolve pass waits for
-// everything queued so far on the other stream (among it the previous
-// draw's resolve), so it overlaps the previous frame's resolve with this
-// frame's setup and still writes framebuffers in call order.
+// setup (and, for a two-pass draw, shading) passes go on that arena's
+// stream at once; its framebuffer pass waits for everything queued so far
+// on the stream of the latest framebuffer work (among it the previous
+// draw's framebuffer pass), so it overlaps earlier frames and still writes
+// framebuffers in call order.
 static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const FrameSize& fs) {
     const int x = ctx->next_arena;
-    FrameArena& fa = x ? ctx->alt : ctx->main;
-    FrameArena& other = x ? ctx->main : ctx->alt;
+    FrameArena& fa = arena_no(ctx, x);
     int r;
-    if (x && !ctx->alt.stream && (r = arena_create(ctx, ctx->err, ctx->alt))) return r;
+    if (x && !fa.stream && (r = arena_create(ctx, ctx->err, fa))) return r;
     if ((r = arena_ensure(ctx, ctx->err, fa, fs))) return r;
     hipEvent_t wait = nullptr;
-    if (other.stream) {
-        wait = ctx->join_ev[x ? 0 : 1];
-        hipError_t e = hipEventRecord(wait, other.stream);
+    if (ctx->last_arena != x) {
+        wait = ctx->join_ev[ctx->last_arena];
+        hipError_t e = hipEventRecord(wait, arena_no(ctx, ctx->last_arena).stream);
         if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
     }
     if ((r = arena_launch(ctx->err, fa, ka, nullptr, wait))) return r;
     fb->clear_pending = false;   // written by this launch, even by a frame that overflows
-    ctx->next_arena = x ^ 1;
+    ctx->last_arena = x;
+    ctx->next_arena = (x + 1) % axr_ctx::kArenas;
     return AXR_OK;
 }
 

@@ -57,15 +57,18 @@ struct axr_ctx {
     axr::ErrSink err;
     axr_config cfg{};
     axr::FrameArena main;
-    // A second arena for asynchronous visibility-mode draws: consecutive such
-    // draws alternate between the two, so one frame's setup runs while the
-    // previous frame's resolve does.  Each draw's framebuffer pass waits for
-    // all work already queued on the other arena's stream, and anything else
-    // queued on main first waits for the second stream (axr::ctx_join), so
-    // framebuffers see the calls in order.
-    axr::FrameArena alt;
-    hipEvent_t join_ev[2] = {};   // recorded on main's / alt's stream for the other to wait on
-    int next_arena = 0;           // arena of the next asynchronous visibility-mode draw
+    // More arenas for asynchronous visibility-mode draws: consecutive such
+    // draws take the arenas in turn (main is arena 0), each on its own
+    // stream, so a frame's setup and shading run while earlier frames still
+    // do.  A draw's framebuffer pass waits for everything queued on the
+    // stream of the previous framebuffer work (which waited for its own
+    // predecessor), and anything queued on main first waits for every other
+    // stream (axr::ctx_join), so framebuffers see the calls in order.
+    static constexpr int kArenas = 3;
+    axr::FrameArena alt[kArenas - 1];
+    hipEvent_t join_ev[kArenas] = {};   // join_ev[k]: recorded on arena k's stream for another to wait on
+    int next_arena = 0;                 // arena of the next asynchronous visibility-mode draw
+    int last_arena = 0;                 // arena whose stream took the latest framebuffer work
     bool profiling = false;
     static constexpr int kEventRing = 64;
     hipEvent_t ev[kEventRing][6] = {};

@@ -1403,12 +1403,14 @@ __global__ __launch_bounds__(256) void k_finish(KArgs ka, Arena ar) {
 }
 
 // k_finish for a width that is a multiple of 4 (every row then starts
-// 16-byte aligned in the 256-byte aligned planes): four pixels per lane,
-// strips of 256 pixels by one bin row, 16-byte loads and stores.
+// 16-byte aligned in the 256-byte aligned planes): one wave per strip of
+// 64 pixels by one bin row, each lane four adjacent pixels of every fourth
+// row, 16-byte loads and stores (each store instruction covers four rows
+// of 256 contiguous bytes).
 template <int T>
 __global__ __launch_bounds__(256) void k_finish4(KArgs ka, Arena ar) {
-    static_assert(256 % T == 0 && T % 4 == 0, "a strip holds whole bins, a lane's pixels one bin");
-    constexpr int RB = T < 8 ? T : 8;   // rows whose slots are loaded together
+    static_assert(64 % T == 0 && T % 4 == 0, "a strip holds whole bins, a lane's pixels one bin");
+    constexpr int RL = T / 4;   // rows per lane
     const bool failed = ar.counters[CNT_ERROR] != 0u;
     const ViewDesc& V = ka.one;
     unsigned long long* vk = ar.vkeys + V.pix_base;
@@ -1416,53 +1418,52 @@ __global__ __launch_bounds__(256) void k_finish4(KArgs ka, Arena ar) {
     const float4 dv = make_float4(ka.clear_depth, ka.clear_depth, ka.clear_depth, ka.clear_depth);
     const ulonglong2 ev2 = make_ulonglong2(EMPTY_KEY, EMPTY_KEY);
     const uint32_t lane = __lane_id();
-    const uint32_t sx_n = ((uint32_t)V.W + 255u) / 256u;
+    const uint32_t cg = lane & 15u, rg = lane >> 4;
+    const uint32_t sx_n = ((uint32_t)V.W + 63u) / 64u;
     const uint32_t nstrips = sx_n * (uint32_t)V.tiles_y;
     const uint32_t nw = gridDim.x * 4u;
     for (uint32_t st = (blockIdx.x * 256u + threadIdx.x) >> 6; st < nstrips; st += nw) {
         const uint32_t sx = st % sx_n, by = st / sx_n;
-        const int px = (int)(sx * 256u + 4u * lane);
+        const int px = (int)(sx * 64u + 4u * cg);
         if (px >= V.W) continue;   // W % 4 == 0: the lane's four pixels are all inside or all outside
         const uint32_t t = V.tile_base + by * (uint32_t)V.tiles_x + (uint32_t)px / (uint32_t)T;
         const bool keys = !failed && ar.tile_count[t] != 0u;
-        const int rows = imin(T, V.H - (int)by * T);
-        const size_t pix0 = (size_t)by * T * (size_t)V.W + (size_t)px;
-        for (int r0 = 0; r0 < rows; r0 += RB) {
-            // all slot loads of RB rows first: the stores below may alias
-            // them for the compiler, which would otherwise wait for each
-            ulonglong2 a[RB], b[RB];
+        const int y0 = (int)by * T;
+        // all slot loads first: the stores below may alias them for the
+        // compiler, which would otherwise wait for each
+        ulonglong2 a[RL], b[RL];
 #pragma unroll
-            for (int r = 0; r < RB; ++r) {
-                a[r] = b[r] = ev2;
-                if (keys && r0 + r < rows) {
-                    const ulonglong2* q = reinterpret_cast<const ulonglong2*>(vk + pix0 + (size_t)(r0 + r) * V.W);
-                    a[r] = q[0];
-                    b[r] = q[1];
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RB; ++r) {
-                if (r0 + r >= rows) break;
-                const size_t pix = pix0 + (size_t)(r0 + r) * V.W;
-                const unsigned long long s0 = a[r].x, s1 = a[r].y, s2 = b[r].x, s3 = b[r].y;
-                const bool any = (s0 & s1 & s2 & s3) != EMPTY_KEY;
-                uint4 c = cv;
-                float4 d = dv;
-                if (any) {
-                    if (s0 != EMPTY_KEY) { c.x = (uint32_t)s0; d.x = __uint_as_float((uint32_t)(s0 >> 32)); }
-                    if (s1 != EMPTY_KEY) { c.y = (uint32_t)s1; d.y = __uint_as_float((uint32_t)(s1 >> 32)); }
-                    if (s2 != EMPTY_KEY) { c.z = (uint32_t)s2; d.z = __uint_as_float((uint32_t)(s2 >> 32)); }
-                    if (s3 != EMPTY_KEY) { c.w = (uint32_t)s3; d.w = __uint_as_float((uint32_t)(s3 >> 32)); }
-                    ulonglong2* q = reinterpret_cast<ulonglong2*>(vk + pix);
-                    q[0] = ev2;
-                    q[1] = ev2;
-                }
-                *reinterpret_cast<uint4*>(V.color + pix) = c;
-                *reinterpret_cast<float4*>(V.depth + pix) = d;
+        for (int k = 0; k < RL; ++k) {
+            a[k] = b[k] = ev2;
+            const int py = y0 + (int)rg + 4 * k;
+            if (keys && py < V.H) {
+                const ulonglong2* q = reinterpret_cast<const ulonglong2*>(vk + (size_t)py * V.W + (size_t)px);
+                a[k] = q[0];
+                b[k] = q[1];
             }
         }
+#pragma unroll
+        for (int k = 0; k < RL; ++k) {
+            const int py = y0 + (int)rg + 4 * k;
+            if (py >= V.H) break;
+            const size_t pix = (size_t)py * V.W + (size_t)px;
+            const unsigned long long s0 = a[k].x, s1 = a[k].y, s2 = b[k].x, s3 = b[k].y;
+            uint4 c = cv;
+            float4 d = dv;
+            if ((s0 & s1 & s2 & s3) != EMPTY_KEY) {
+                if (s0 != EMPTY_KEY) { c.x = (uint32_t)s0; d.x = __uint_as_float((uint32_t)(s0 >> 32)); }
+                if (s1 != EMPTY_KEY) { c.y = (uint32_t)s1; d.y = __uint_as_float((uint32_t)(s1 >> 32)); }
+                if (s2 != EMPTY_KEY) { c.z = (uint32_t)s2; d.z = __uint_as_float((uint32_t)(s2 >> 32)); }
+                if (s3 != EMPTY_KEY) { c.w = (uint32_t)s3; d.w = __uint_as_float((uint32_t)(s3 >> 32)); }
+                ulonglong2* q = reinterpret_cast<ulonglong2*>(vk + pix);
+                q[0] = ev2;
+                q[1] = ev2;
+            }
+            *reinterpret_cast<uint4*>(V.color + pix) = c;
+            *reinterpret_cast<float4*>(V.depth + pix) = d;
+        }
         // every lane of a bin read its flag before the loop's loads returned
-        if (keys && (uint32_t)px % (uint32_t)T == 0u) ar.tile_count[t] = 0u;
+        if (keys && rg == 0u && (uint32_t)px % (uint32_t)T == 0u) ar.tile_count[t] = 0u;
     }
 }
 
@@ -1543,7 +1544,7 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
         if (ev) (void)hipEventRecord(ev[3], st);
         if (wait_last) (void)hipStreamWaitEvent(st, wait_last, 0);
         if (ka.one.W % 4 == 0) {
-            const uint32_t strips = (((uint32_t)ka.one.W + 255u) / 256u) * (uint32_t)ka.one.tiles_y;
+            const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
             k_finish4<T><<<imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
         } else {
             const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;

@@ -219,8 +219,8 @@ def rotated(sc, deg, flags=0):
 
 
 def test_async_draws_keep_call_order(ctx):
-    """Asynchronous draws alternate between two frame arenas and overlap;
-    draws into one framebuffer must still land in call order: six rotated
+    """Asynchronous draws take the frame arenas in turn and overlap; draws
+    into one framebuffer must still land in call order: six rotated
     copies drawn without clears in between (each depth-tested against the
     ones before), interleaved with draws into a second framebuffer, a
     synchronous draw and a clear."""
