@@ -1,6 +1,7 @@
 // capi.hip - host side of the C ABI declared in include/axiomr.h: context,
 // device resources, the frame arena, drawMesh, the graph-captured batch.
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -207,6 +208,42 @@ static int arena_reset(ErrSink& err, FrameArena& fa) {
     return e == hipSuccess ? AXR_OK : hip_fail(err, e, "reset counters");
 }
 
+// Signal memory for the framebuffer handoff between arena streams: one
+// 8-byte signal per frame arena, waited on with hipStreamWaitValue64 and
+// written with hipStreamWriteValue64.  Signals are never freed (the runtime
+// serves them from memory the host and the command processor share);
+// an arena that is destroyed returns its signal to a per-device free list
+// for the next one.  Every value written comes from one process-wide
+// counter, so a signal's value only grows and a wait for "at least v" is
+// met by the write of v and by nothing written before it.
+namespace {
+std::mutex g_sig_mu;
+std::vector<std::pair<int, uint64_t*>> g_sig_free;   // (device, signal)
+std::atomic<uint64_t> g_sig_value{0};
+
+uint64_t* sig_acquire(int device) {
+    std::lock_guard<std::mutex> lk(g_sig_mu);
+    for (size_t i = 0; i < g_sig_free.size(); ++i)
+        if (g_sig_free[i].first == device) {
+            uint64_t* p = g_sig_free[i].second;
+            g_sig_free.erase(g_sig_free.begin() + (long)i);
+            return p;
+        }
+    int can = 0;
+    if (hipDeviceGetAttribute(&can, hipDeviceAttributeCanUseStreamWaitValue, device) != hipSuccess || !can)
+        return nullptr;
+    void* p = nullptr;
+    if (hipExtMallocWithFlags(&p, 8, hipMallocSignalMemory) != hipSuccess) return nullptr;
+    return (uint64_t*)p;
+}
+
+void sig_release(int device, uint64_t* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_sig_mu);
+    g_sig_free.emplace_back(device, p);
+}
+}  // namespace
+
 int arena_create(axr_ctx* ctx, ErrSink& err, FrameArena& fa) {
     hipError_t e = hipStreamCreateWithFlags(&fa.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&fa.host_counters, CNT_COUNT * 4, hipHostMallocDefault);
@@ -216,6 +253,13 @@ int arena_create(axr_ctx* ctx, ErrSink& err, FrameArena& fa) {
     fs.tiles = 1024;
     fs.wave_cap = (uint32_t)ctx->cfg.bin_capacity;
     int r = arena_ensure(ctx, err, fa, fs);
+    // null: this arena's draws hand over with events.  The value starts at
+    // 0 (arena_reset waits for the stream).
+    fa.sig = sig_acquire(ctx->device);
+    if (fa.sig && hipStreamWriteValue64(fa.stream, fa.sig, 0, 0) != hipSuccess) {
+        sig_release(ctx->device, fa.sig);
+        fa.sig = nullptr;
+    }
     return r ? r : arena_reset(err, fa);
 }
 
@@ -241,6 +285,7 @@ void arena_destroy(axr_ctx* ctx, FrameArena& fa) {
     for (DevBuf* b : bufs) free_buf(ctx, *b);
     if (fa.host_counters) (void)hipHostFree(fa.host_counters);
     if (fa.stream) (void)hipStreamDestroy(fa.stream);
+    sig_release(ctx->device, fa.sig);
     fa = FrameArena{};
 }
 
@@ -258,11 +303,11 @@ static int arena_prep_if_dirty(ErrSink& err, FrameArena& fa) {
     return AXR_OK;
 }
 
-int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev, hipEvent_t wait_last) {
+int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev, const FbWait& wait) {
     int r = arena_prep_if_dirty(err, fa);
     if (r) return r;
     if (ev) (void)hipEventRecord(ev[0], fa.stream);
-    hipError_t e = launch_frame(ka, fa.a, fa.stream, ev ? ev + 1 : nullptr, wait_last);
+    hipError_t e = launch_frame(ka, fa.a, fa.stream, ev ? ev + 1 : nullptr, wait);
     return e == hipSuccess ? AXR_OK : hip_fail(err, e, "kernel launch");
 }
 
@@ -296,6 +341,7 @@ int ctx_join(axr_ctx* ctx) {
     }
     if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
     ctx->last_arena = 0;
+    ctx->last_value = 0;
     return AXR_OK;
 }
 
@@ -758,14 +804,28 @@ static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const Frame
     int r;
     if (x && !fa.stream && (r = arena_create(ctx, ctx->err, fa))) return r;
     if ((r = arena_ensure(ctx, ctx->err, fa, fs))) return r;
-    hipEvent_t wait = nullptr;
+    FbWait wait;
     if (ctx->last_arena != x) {
-        wait = ctx->join_ev[ctx->last_arena];
-        hipError_t e = hipEventRecord(wait, arena_no(ctx, ctx->last_arena).stream);
-        if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
+        FrameArena& last = arena_no(ctx, ctx->last_arena);
+        if (ctx->last_value && last.sig) {
+            wait.sig = last.sig;
+            wait.value = ctx->last_value;
+        } else {
+            wait.ev = ctx->join_ev[ctx->last_arena];
+            hipError_t e = hipEventRecord(wait.ev, last.stream);
+            if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
+        }
     }
     if ((r = arena_launch(ctx->err, fa, ka, nullptr, wait))) return r;
     fb->clear_pending = false;   // written by this launch, even by a frame that overflows
+    // the end of this draw's framebuffer pass, for the next draw to wait on
+    ctx->last_value = 0;
+    if (fa.sig) {
+        const uint64_t v = ++g_sig_value;
+        hipError_t e = hipStreamWriteValue64(fa.stream, fa.sig, v, 0);
+        if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
+        ctx->last_value = v;
+    }
     ctx->last_arena = x;
     ctx->next_arena = (x + 1) % axr_ctx::kArenas;
     return AXR_OK;

@@ -12,11 +12,19 @@
 
 namespace axr {
 
-// wait_last (optional): an event the stream waits for before the pass that
-// writes framebuffers (k_resolve / k_raster); the passes before it only touch
-// the frame arena and the meshes.
-hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev,
-                        hipEvent_t wait_last = nullptr);
+// What a frame's framebuffer pass waits for first: an event, or an 8-byte
+// signal reaching at least `value` (stream memory wait, about a third of the
+// host time of an event record + wait); nothing when both are empty.
+struct FbWait {
+    hipEvent_t ev = nullptr;
+    uint64_t* sig = nullptr;
+    uint64_t value = 0;
+};
+
+// wait (optional): what the stream waits for before the pass that writes
+// framebuffers (k_resolve / k_finish / k_raster); the passes before it only
+// touch the frame arena and the meshes.
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& wait = {});
 hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st);
 hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st);
 hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
@@ -44,6 +52,7 @@ struct FrameArena {
     uint64_t gen = 0;          // bumped whenever a buffer moves: captured graphs are stale
     bool dirty = true;         // bin counts / counters need a k_prep before the next frame
     uint32_t* host_counters = nullptr;   // pinned, CNT_COUNT words
+    uint64_t* sig = nullptr;   // signal memory its stream writes after a framebuffer pass (null: use events)
 };
 
 }  // namespace axr
@@ -69,6 +78,8 @@ struct axr_ctx {
     hipEvent_t join_ev[kArenas] = {};   // join_ev[k]: recorded on arena k's stream for another to wait on
     int next_arena = 0;                 // arena of the next asynchronous visibility-mode draw
     int last_arena = 0;                 // arena whose stream took the latest framebuffer work
+    uint64_t last_value = 0;            // nonzero: that work was a draw whose stream then wrote this
+                                        // value to its arena's signal (0: wait with an event)
     bool profiling = false;
     static constexpr int kEventRing = 64;
     hipEvent_t ev[kEventRing][6] = {};
@@ -141,7 +152,7 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs
 int arena_read_counters(ErrSink& err, FrameArena& fa);
 // k_prep if the arena is fresh or dirty, then the frame's kernels; ev (may
 // be null) receives the 5 stage events.
-int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev, hipEvent_t wait_last = nullptr);
+int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev, const FbWait& wait = {});
 // After arena_read_counters: AXR_OK, or grow the arena to the reported
 // high-water marks, reset it and return AXR_ERR_OVERFLOW.
 int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs);

@@ -21,6 +21,7 @@
 // strict-less depth test over triangles in submission order.
 #include "axr_types.hpp"
 #include "shade.hpp"
+#include "internal.hpp"
 
 #pragma clang fp contract(off)
 
@@ -1521,8 +1522,14 @@ constexpr int RESOLVE_MAX_BLOCKS = 16384;
 // Visibility mode: k_vis, k_clip, k_big, k_resolve (or k_shade, k_finish
 // for a two-pass draw).  Events: after k_vis, k_clip, k_big, (nothing or
 // k_shade), k_resolve (or k_finish).
+static hipError_t fb_wait(hipStream_t st, const FbWait& w) {
+    if (w.sig) return hipStreamWaitValue64(st, w.sig, w.value, hipStreamWaitValueGte);
+    if (w.ev) return hipStreamWaitEvent(st, w.ev, 0);
+    return hipSuccess;
+}
+
 template <int T, bool BATCH>
-static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t wait_last) {
+static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& wait) {
     const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
     if (vis_blocks) k_vis<T, BATCH><<<vis_blocks, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[0], st);
@@ -1542,7 +1549,8 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
         default: k_shade<T, SHADER_FLAT_COLOR><<<rb, 256, 0, st>>>(ka, ar); break;
         }
         if (ev) (void)hipEventRecord(ev[3], st);
-        if (wait_last) (void)hipStreamWaitEvent(st, wait_last, 0);
+        hipError_t e = fb_wait(st, wait);
+        if (e != hipSuccess) return e;
         if (ka.one.W % 4 == 0) {
             const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
             k_finish4<T><<<imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
@@ -1555,7 +1563,8 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     }
     if (ev) (void)hipEventRecord(ev[3], st);
     // the framebuffer pass waits for what other streams were asked to do first
-    if (wait_last) (void)hipStreamWaitEvent(st, wait_last, 0);
+    hipError_t e = fb_wait(st, wait);
+    if (e != hipSuccess) return e;
     switch (ka.shader) {
     case SHADER_FLAT: k_resolve<T, BATCH, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
     case SHADER_PHONG: k_resolve<T, BATCH, SHADER_PHONG><<<rb, 256, 0, st>>>(ka, ar); break;
@@ -1571,8 +1580,8 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
 
 template <int T, int NT, bool BATCH>
 static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev,
-                                 hipEvent_t wait_last) {
-    if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, BATCH>(ka, ar, st, ev, wait_last);
+                                 const FbWait& wait) {
+    if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, BATCH>(ka, ar, st, ev, wait);
     const uint32_t ntiles = ka.total_tiles;
     const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
     if (setup_blocks) k_setup<T, BATCH><<<setup_blocks, 256, 0, st>>>(ka, ar);
@@ -1593,7 +1602,8 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
     k_scatter<<<sblocks, 256, 0, st>>>(ar, nwaves);
     if (ev) (void)hipEventRecord(ev[3], st);
     const uint32_t rblocks = ((imax(1, (int)ntiles) + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
-    if (wait_last) (void)hipStreamWaitEvent(st, wait_last, 0);
+    hipError_t e = fb_wait(st, wait);
+    if (e != hipSuccess) return e;
     switch (ka.shader) {
     case SHADER_FLAT: k_raster<T, NT, BATCH, SHADER_FLAT><<<rblocks, NT, 0, st>>>(ka, ar); break;
     case SHADER_PHONG: k_raster<T, NT, BATCH, SHADER_PHONG><<<rblocks, NT, 0, st>>>(ka, ar); break;
@@ -1608,7 +1618,7 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
 }
 
 template <int T, bool BATCH>
-static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t w) {
+static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& w) {
     switch (ka.raster_threads) {
     case 64: return launch_frame_t<T, 64, BATCH>(ka, ar, st, ev, w);
     case 128: return launch_frame_t<T, 128, BATCH>(ka, ar, st, ev, w);
@@ -1617,7 +1627,7 @@ static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t 
 }
 
 template <bool BATCH>
-static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t w) {
+static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& w) {
     switch (ka.tile_size) {
     case 8: return launch_frame_nt<8, BATCH>(ka, ar, st, ev, w);
     case 32: return launch_frame_nt<32, BATCH>(ka, ar, st, ev, w);
@@ -1625,8 +1635,8 @@ static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t s
     }
 }
 
-hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, hipEvent_t wait_last) {
-    return ka.table ? launch_frame_b<true>(ka, ar, st, ev, wait_last) : launch_frame_b<false>(ka, ar, st, ev, wait_last);
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& wait) {
+    return ka.table ? launch_frame_b<true>(ka, ar, st, ev, wait) : launch_frame_b<false>(ka, ar, st, ev, wait);
 }
 
 hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
