@@ -73,7 +73,7 @@ struct axr_ctx {
     // stream of the previous framebuffer work (which waited for its own
     // predecessor), and anything queued on main first waits for every other
     // stream (axr::ctx_join), so framebuffers see the calls in order.
-    static constexpr int kArenas = 3;
+    static constexpr int kArenas = 4;
     axr::FrameArena alt[kArenas - 1];
     hipEvent_t join_ev[kArenas] = {};   // join_ev[k]: recorded on arena k's stream for another to wait on
     int next_arena = 0;                 // arena of the next asynchronous visibility-mode draw
