@@ -4,8 +4,8 @@ workgroup (64/128/256), per-wave bin-pair capacity (64..1024, binned mode)
 and the one-lane footprint threshold.  None of them changes a pixel; they
 only move time between the kernels.
 
-Coordinate descent: raster mode x bin edge x workgroup size first (18
-points), then the bin capacity (binned mode only), then the footprint
+Coordinate descent, two rounds: raster mode x bin edge x workgroup size
+(18 points), then the bin capacity (binned mode only), then the footprint
 threshold, each point timed as the median of a few synchronised runs of the
 caller's frame function.  Results are cached in a JSON file keyed by device
 and workload name.
@@ -25,10 +25,26 @@ FOOTPRINTS = (8, 16, 32, 64, 128, 256)
 DEFAULT_CACHE = Path(__file__).resolve().parent / "tuning" / "mi355x.json"
 
 
+def _warm(ctx, frame: Callable[[], None], n: int, tries: int = 8) -> None:
+    """n frames and a wait, again while a frame arena reports an overflow (it
+    is grown at the wait; the frames since the previous wait were not drawn)."""
+    from ._native import AXR_ERR_OVERFLOW, AxrError
+    for attempt in range(tries):
+        for _ in range(n):
+            frame()
+        try:
+            ctx.sync()
+            return
+        except AxrError as e:
+            if e.code != AXR_ERR_OVERFLOW or attempt == tries - 1:
+                raise
+
+
 def time_config(ctx, frame: Callable[[], None], reps: int = 3, inner: int = 2) -> float:
-    """Median seconds of `inner` frames, over `reps` runs, after a warm-up."""
-    frame()
-    ctx.sync()
+    """Median seconds of `inner` frames, over `reps` runs, after a warm-up of
+    as many frames (asynchronous frames overlap, so a short run would time
+    the pipeline filling instead of its steady state)."""
+    _warm(ctx, frame, inner)
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -57,29 +73,31 @@ def tune(ctx, make_frame: Callable[[], Callable[[], None]], log: Optional[Callab
         return t
 
     best_t = run(best)
-    start = dict(best)
-    for mode in MODES:
-        for tile in TILES:
-            for nt in THREADS:
-                cfg = {**start, "raster_mode": mode, "tile_size": tile, "raster_threads": nt}
-                if cfg == start:
-                    continue
-                t = run(cfg)
-                if t < best_t:
-                    best, best_t = cfg, t
-    if best.get("raster_mode", 0) == 0:
-        for cap in CAPACITIES:
-            cfg = {**best, "bin_capacity": cap}
-            if cfg != best:
-                t = run(cfg)
-                if t < best_t:
-                    best, best_t = cfg, t
-    for fp in FOOTPRINTS:
-        cfg = {**best, "small_footprint": fp}
-        if cfg != best:
-            t = run(cfg)
-            if t < best_t:
-                best, best_t = cfg, t
+    seen = {tuple(sorted(best.items()))}
+
+    def consider(cfg):
+        nonlocal best, best_t
+        key = tuple(sorted(cfg.items()))
+        if key in seen:
+            return
+        seen.add(key)
+        t = run(cfg)
+        if t < best_t:
+            best, best_t = cfg, t
+
+    # two rounds: the second sweeps the launch shape again under the
+    # footprint and capacity the first one chose
+    for _ in range(2):
+        start = dict(best)
+        for mode in MODES:
+            for tile in TILES:
+                for nt in THREADS:
+                    consider({**start, "raster_mode": mode, "tile_size": tile, "raster_threads": nt})
+        if best.get("raster_mode", 0) == 0:
+            for cap in CAPACITIES:
+                consider({**best, "bin_capacity": cap})
+        for fp in FOOTPRINTS:
+            consider({**best, "small_footprint": fp})
     ctx.set_config(**best)
     return {"best": best, "seconds": best_t, "trials": trials}
 

@@ -4,6 +4,7 @@
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include "internal.hpp"
@@ -221,7 +222,17 @@ std::mutex g_sig_mu;
 std::vector<std::pair<int, uint64_t*>> g_sig_free;   // (device, signal)
 std::atomic<uint64_t> g_sig_value{0};
 
+// AXR_STREAM_WAIT_VALUE=0: hand over with events instead.
+bool sig_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("AXR_STREAM_WAIT_VALUE");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 uint64_t* sig_acquire(int device) {
+    if (!sig_enabled()) return nullptr;
     std::lock_guard<std::mutex> lk(g_sig_mu);
     for (size_t i = 0; i < g_sig_free.size(); ++i)
         if (g_sig_free[i].first == device) {
@@ -321,16 +332,39 @@ int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const Frame
     need.pairs = grown(c[CNT_NEED_PAIRS], 1, 4);
     need.big_recs = grown(c[CNT_NEED_BIG_RECS], 1, 2);
     need.big_items = grown(c[CNT_NEED_BIG_ITEMS], 1, 2);
+    // every arena of the context is grown to at least this on its next draw
+    ArenaNeeds& fl = ctx->need_floor;
+    fl.clip_faces = std::max(fl.clip_faces, need.clip_faces);
+    fl.clip_recs = std::max(fl.clip_recs, need.clip_recs);
+    fl.pairs = std::max(fl.pairs, need.pairs);
+    fl.big_recs = std::max(fl.big_recs, need.big_recs);
+    fl.big_items = std::max(fl.big_items, need.big_items);
+    const uint32_t seen[5] = {c[CNT_NEED_CLIP_FACES], c[CNT_NEED_CLIP_RECS], c[CNT_NEED_PAIRS], c[CNT_NEED_BIG_RECS],
+                              c[CNT_NEED_BIG_ITEMS]};   // the reset below clears the counters
     int r = arena_ensure(ctx, err, fa, fs, need);
     if (r) return r;
     if ((r = arena_reset(err, fa))) return r;
     return fail(err, AXR_ERR_OVERFLOW, "frame arena overflow (clip faces %u, clip records %u, bin pairs %u, "
                 "large triangles %u, large-triangle items %u); arena grown, frames since the last sync were not drawn",
-                c[CNT_NEED_CLIP_FACES], c[CNT_NEED_CLIP_RECS], c[CNT_NEED_PAIRS], c[CNT_NEED_BIG_RECS],
-                c[CNT_NEED_BIG_ITEMS]);
+                seen[0], seen[1], seen[2], seen[3], seen[4]);
 }
 
 static FrameArena& arena_no(axr_ctx* ctx, int k) { return k ? ctx->alt[k - 1] : ctx->main; }
+
+// Tools that collect hardware counters serialize kernel dispatches across
+// queues, and work spread over streams that wait on each other can then
+// stall; under them (or with AXR_OVERLAP=0) every draw goes to the
+// context's own stream, one frame after the other.
+static bool overlap_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("AXR_OVERLAP");
+        if (v && v[0] == '0') return false;
+        for (const char* k : {"ROCPROF_COUNTER_COLLECTION", "ROCPROF_COUNTERS", "HSA_TOOLS_LIB", "ROCP_TOOL_LIB"})
+            if (std::getenv(k)) return false;
+        return true;
+    }();
+    return on;
+}
 
 int ctx_join(axr_ctx* ctx) {
     hipError_t e = hipSuccess;
@@ -749,13 +783,16 @@ int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids) {
     const size_t n = (size_t)fb->w * fb->h;
     int r = fb_settle(fb);
     if (r || (r = ctx_sync_all(ctx))) return r;
+    if (ids && !fb->ids) return fail(ctx->err, AXR_ERR_INVALID, "framebuffer has no id plane");
+    // copies on the context's own stream: a copy on the null stream would
+    // give the process one more hardware queue to share with the frame
+    // arenas' streams, which then slows every later overlapped draw
+    const hipStream_t st = ctx->main.stream;
     hipError_t e = hipSuccess;
-    if (e == hipSuccess && bgra) e = hipMemcpy(bgra, fb->color, n * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && depth) e = hipMemcpy(depth, fb->depth, n * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && ids) {
-        if (!fb->ids) return fail(ctx->err, AXR_ERR_INVALID, "framebuffer has no id plane");
-        e = hipMemcpy(ids, fb->ids, n * 4, hipMemcpyDeviceToHost);
-    }
+    if (e == hipSuccess && bgra) e = hipMemcpyAsync(bgra, fb->color, n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && depth) e = hipMemcpyAsync(depth, fb->depth, n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && ids) e = hipMemcpyAsync(ids, fb->ids, n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e == hipSuccess ? AXR_OK : hip_fail(ctx->err, e, "framebuffer read");
 }
 
@@ -803,7 +840,7 @@ static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const Frame
     FrameArena& fa = arena_no(ctx, x);
     int r;
     if (x && !fa.stream && (r = arena_create(ctx, ctx->err, fa))) return r;
-    if ((r = arena_ensure(ctx, ctx->err, fa, fs))) return r;
+    if ((r = arena_ensure(ctx, ctx->err, fa, fs, ctx->need_floor))) return r;
     FbWait wait;
     if (ctx->last_arena != x) {
         FrameArena& last = arena_no(ctx, ctx->last_arena);
@@ -864,10 +901,11 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     // ids shades apart from its framebuffer writes (also when profiled, so
     // the stage times are those of the same kernels)
     ka.two_pass = async && ka.clear && !fb->ids;
-    if (async && ka.mode == RASTER_VISIBILITY && !ctx->profiling) return draw_pipelined(ctx, fb, ka, fs);
+    if (async && ka.mode == RASTER_VISIBILITY && !ctx->profiling && overlap_enabled())
+        return draw_pipelined(ctx, fb, ka, fs);
     int r;
     if ((fb->clear_pending && !ka.clear && (r = fb_settle(fb))) || (r = ctx_join(ctx)) ||
-        (r = arena_ensure(ctx, ctx->err, ctx->main, fs)))
+        (r = arena_ensure(ctx, ctx->err, ctx->main, fs, ctx->need_floor)))
         return r;
     for (int attempt = 0; attempt < 8; ++attempt) {
         if ((r = ctx_launch(ctx, ka))) return r;

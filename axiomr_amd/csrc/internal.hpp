@@ -41,6 +41,11 @@ struct ErrSink {
     std::string msg;
 };
 
+// Capacities an overflowing frame reported (0: no requirement).
+struct ArenaNeeds {
+    uint64_t clip_faces = 0, clip_recs = 0, pairs = 0, big_recs = 0, big_items = 0;
+};
+
 // The per-frame working memory of the binning pipeline plus the stream its
 // kernels run on.  A context owns one; each lane of a render server owns one,
 // so batches on different lanes overlap on the device.
@@ -78,6 +83,7 @@ struct axr_ctx {
     hipEvent_t join_ev[kArenas] = {};   // join_ev[k]: recorded on arena k's stream for another to wait on
     int next_arena = 0;                 // arena of the next asynchronous visibility-mode draw
     int last_arena = 0;                 // arena whose stream took the latest framebuffer work
+    axr::ArenaNeeds need_floor;         // the largest overflow needs reported: every arena is grown to them
     uint64_t last_value = 0;            // nonzero: that work was a draw whose stream then wrote this
                                         // value to its arena's signal (0: wait with an event)
     bool profiling = false;
@@ -140,9 +146,6 @@ struct FrameSize {
     uint32_t wave_cap = 128; // pairs per setup-wave region (binned mode)
 };
 // High-water marks the kernels reported after an overflow.
-struct ArenaNeeds {
-    uint64_t clip_faces = 0, clip_recs = 0, pairs = 0, big_recs = 0, big_items = 0;
-};
 FrameSize frame_size(const KArgs& ka, uint32_t wave_cap);
 
 int arena_create(axr_ctx* ctx, ErrSink& err, FrameArena& fa);

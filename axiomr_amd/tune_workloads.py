@@ -36,7 +36,10 @@ def tune_plain(ctx):
             L.axr_draw(ctx.handle, fb.handle, gs.mesh.handle, C.byref(p))
         return frame
 
-    res = AT.tune(ctx, make_frame, log, reps=5, inner=20)
+    cached = AT.load("plain")
+    if cached:
+        ctx.set_config(**cached["config"])
+    res = AT.tune(ctx, make_frame, log, reps=5, inner=200)
     fb.close()
     gs.close()
     return res

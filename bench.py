@@ -82,6 +82,21 @@ def apply_config(ctx, workload, tile, use_tuned):
     return ctx.config()
 
 
+def warm(ctx, enqueue, tries=8):
+    """Run `enqueue` (asynchronous frames) and wait for it.  A frame arena
+    that overflowed is reported at the wait, grown and reset, and the frames
+    since the previous wait were not drawn: run again until none overflows."""
+    from axiomr_amd._native import AXR_ERR_OVERFLOW, AxrError
+    for attempt in range(tries):
+        enqueue()
+        try:
+            ctx.sync()
+            return
+        except AxrError as e:
+            if e.code != AXR_ERR_OVERFLOW or attempt == tries - 1:
+                raise
+
+
 def stage_names(cfg, two_pass=False):
     """Names of the five event-timed stages in the context's raster mode;
     two_pass: visibility-mode draws that resolve in two launches (an
@@ -117,9 +132,7 @@ def run_plain(scene, steps, warmup, tile, use_tuned):
     _, depth, _ = fb.read()
     covered = int(np.isfinite(depth).sum())
 
-    for _ in range(warmup):
-        frame()
-    ctx.sync()
+    warm(ctx, lambda: [frame() for _ in range(max(1, warmup))])
     t0 = time.perf_counter()
     for _ in range(steps):
         frame()
@@ -161,9 +174,7 @@ def run_batch(wl, reps, tile, use_tuned):
         if gb is None:
             gb = H.GpuBatch(ctx, wl, with_ids=False)
         n = len(wl.views)
-        for i in range(n):
-            gb.draw_solo(i, flags=1)
-        ctx.sync()
+        warm(ctx, lambda: [gb.draw_solo(i, flags=1) for i in range(n)])
         t0 = time.perf_counter()
         for _ in range(reps):
             for i in range(n):
