@@ -183,6 +183,15 @@ def run_batch(wl, reps, tile, use_tuned):
         dt = (time.perf_counter() - t0) / reps
         if plain_s is None or dt < plain_s:
             plain_s, plain_cfg = dt, cfg
+    # the same views through the synchronous drop-in drawMesh (each draw
+    # returns when its frame is written, as the reference's does), plain config
+    apply_config(ctx, "plain", tile, use_tuned)
+    for i in range(n):
+        gb.draw_solo(i)
+    t0 = time.perf_counter()
+    for i in range(n):
+        gb.draw_solo(i)
+    sync_s = time.perf_counter() - t0
     cfg = apply_config(ctx, "batch", tile, use_tuned)
     gb.rebuild_batch()
     # batched: one graph launch draws (and clears) every view
@@ -202,6 +211,7 @@ def run_batch(wl, reps, tile, use_tuned):
     gb.close()
     ctx.close()
     return dict(views=n, plain_views_per_s=n / plain_s, batch_views_per_s=n / batch_s, speedup=plain_s / batch_s,
+                sync_views_per_s=n / sync_s, speedup_vs_sync=sync_s / batch_s,
                 batch_ms=batch_s * 1e3, plain_ms=plain_s * 1e3, stages_ms=stages,
                 faces=wl.total_faces, pixels=wl.total_pixels, config=cfg, plain_config=plain_cfg)
 
@@ -497,14 +507,17 @@ def main():
         cb = None if args.no_cpu else cpu_batch(wl)
         if cb:
             log(f"[bench] batch on the CPU checker: {cb['views_per_s']:.1f} views/s ({cb['threads']} threads)")
-        log(f"[bench] batch: plain {b['plain_views_per_s']:.0f} views/s, batched {b['batch_views_per_s']:.0f} views/s, "
-            f"x{b['speedup']:.2f}; stages {b['stages_ms']}")
+        log(f"[bench] batch: plain {b['plain_views_per_s']:.0f} views/s (queued), {b['sync_views_per_s']:.0f} views/s "
+            f"(synchronous drawMesh), batched {b['batch_views_per_s']:.0f} views/s, x{b['speedup']:.2f} / "
+            f"x{b['speedup_vs_sync']:.2f}; stages {b['stages_ms']}")
         batch = {"workload": f"{len(wl.views)} views of 32 displaced icospheres (5k-500k triangles, log-uniform), "
                              "256x256..1920x1080 log-uniform, Phong + textures, one graph launch per batch",
                  "views": b["views"], "triangles": b["faces"], "pixels": b["pixels"],
                  "batched_views_per_s": round(b["batch_views_per_s"], 1),
                  "plain_views_per_s": round(b["plain_views_per_s"], 1),
                  "speedup_vs_plain": round(b["speedup"], 3),
+                 "sync_views_per_s": round(b["sync_views_per_s"], 1),
+                 "speedup_vs_sync": round(b["speedup_vs_sync"], 3),
                  "batched_gtri_per_s": round(b["faces"] / (b["batch_ms"] * 1e-3) / 1e9, 3),
                  "batch_ms": round(b["batch_ms"], 3), "stages_ms": {k: round(v, 4) for k, v in b["stages_ms"].items()},
                  "tunables": b["config"], "plain_tunables": b["plain_config"],
