@@ -1339,22 +1339,27 @@ __global__ __launch_bounds__(256) void k_shade(KArgs ka, Arena ar) {
     }
     const ViewDesc& V = ka.one;
     unsigned long long* vk = ar.vkeys + V.pix_base;
+    // one wave per 64 pixels of a bin (SUB waves per bin), one pixel per
+    // lane: a bin edge of 16 or 32 gives the pass as many waves as one of 8
+    constexpr uint32_t SUB = (uint32_t)(T * T + 63) / 64;
+    const uint32_t nunits = ka.total_tiles * SUB;
+    const int lane = (int)__lane_id();
     const uint32_t nw = gridDim.x * 4u;
-    for (uint32_t t = (blockIdx.x * 256u + threadIdx.x) >> 6; t < ka.total_tiles; t += nw) {
+    for (uint32_t u = (blockIdx.x * 256u + threadIdx.x) >> 6; u < nunits; u += nw) {
+        const uint32_t t = u / SUB;
         if (ar.tile_count[t] == 0u) continue;
+        const int p = (int)(u % SUB) * 64 + lane;   // row-major within the bin
         const uint32_t local = t - V.tile_base;
-        const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
-        for (int p = (int)__lane_id(); p < T * T; p += 64) {
-            const int px = ox + (p % T), py = oy + (p / T);
-            if (px >= V.W || py >= V.H) continue;
-            const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
-            const unsigned long long k = vk[pix];
-            if (k == EMPTY_KEY) continue;
-            float z;
-            uint32_t bgra;
-            const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra);
-            vk[pix] = drawn ? ((unsigned long long)__float_as_uint(z) << 32) | bgra : EMPTY_KEY;
-        }
+        const int px = (int)(local % (uint32_t)V.tiles_x) * T + (p % T);
+        const int py = (int)(local / (uint32_t)V.tiles_x) * T + (p / T);
+        if (p >= T * T || px >= V.W || py >= V.H) continue;
+        const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
+        const unsigned long long k = vk[pix];
+        if (k == EMPTY_KEY) continue;
+        float z;
+        uint32_t bgra;
+        const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra);
+        vk[pix] = drawn ? ((unsigned long long)__float_as_uint(z) << 32) | bgra : EMPTY_KEY;
     }
 }
 
@@ -1540,13 +1545,15 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     const int rb = imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS));
     if (!BATCH && ka.two_pass && ka.clear && !ka.one.ids && ka.shader >= 0 && ka.shader < SHADER_COUNT) {
         // shade at once, write the framebuffer after the earlier draws
+        const uint64_t units = (uint64_t)ka.total_tiles * (uint64_t)((T * T + 63) / 64);
+        const int sb = (int)std::min<uint64_t>(std::max<uint64_t>(1, (units + 3) / 4), RESOLVE_MAX_BLOCKS);
         switch (ka.shader) {
-        case SHADER_FLAT: k_shade<T, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
-        case SHADER_PHONG: k_shade<T, SHADER_PHONG><<<rb, 256, 0, st>>>(ka, ar); break;
-        case SHADER_PBR: k_shade<T, SHADER_PBR><<<rb, 256, 0, st>>>(ka, ar); break;
-        case SHADER_GOURAUD: k_shade<T, SHADER_GOURAUD><<<rb, 256, 0, st>>>(ka, ar); break;
-        case SHADER_PHONG_POINT: k_shade<T, SHADER_PHONG_POINT><<<rb, 256, 0, st>>>(ka, ar); break;
-        default: k_shade<T, SHADER_FLAT_COLOR><<<rb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_FLAT: k_shade<T, SHADER_FLAT><<<sb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_PHONG: k_shade<T, SHADER_PHONG><<<sb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_PBR: k_shade<T, SHADER_PBR><<<sb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_GOURAUD: k_shade<T, SHADER_GOURAUD><<<sb, 256, 0, st>>>(ka, ar); break;
+        case SHADER_PHONG_POINT: k_shade<T, SHADER_PHONG_POINT><<<sb, 256, 0, st>>>(ka, ar); break;
+        default: k_shade<T, SHADER_FLAT_COLOR><<<sb, 256, 0, st>>>(ka, ar); break;
         }
         if (ev) (void)hipEventRecord(ev[3], st);
         hipError_t e = fb_wait(st, wait);

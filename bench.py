@@ -489,7 +489,8 @@ def main():
     elif "resolve" in stages:
         dominant = "resolve"
         kbytes = resolve_bytes(W, H, r["covered"], ntex)
-        kname = "k_shade+k_finish" if "shade" in r["stages_ms"] else "k_resolve"
+        # k_finish4 (16-byte writes) when the width is a multiple of 4
+        kname = ("k_shade+" + ("k_finish4" if W % 4 == 0 else "k_finish")) if "shade" in r["stages_ms"] else "k_resolve"
     else:
         dominant = "raster"
         kbytes, kname = raster_bytes(r["covered"], ntex), "k_raster"
