@@ -10,9 +10,12 @@
  * the last failure on a context is available from axr_last_error.
  *
  * Threading: a context is used by one host thread at a time.  Work is
- * enqueued on the context's HIP stream; axr_draw is synchronous unless
+ * enqueued on the context's HIP streams (four, one per frame arena;
+ * asynchronous visibility-mode draws take them in turn and overlap, while
+ * framebuffer writes keep call order); axr_draw is synchronous unless
  * AXR_DRAW_ASYNC is set, in which case completion (and any arena overflow) is
- * reported by axr_sync.
+ * reported by axr_sync.  With AXR_OVERLAP=0 in the environment, or under a
+ * profiler that collects hardware counters, draws do not overlap.
  */
 #ifndef AXIOMR_H
 #define AXIOMR_H
