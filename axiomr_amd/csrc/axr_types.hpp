@@ -204,6 +204,10 @@ struct KArgs {
     int32_t two_pass = 0;       // plain visibility-mode draw with a clear and no id
                                 // plane: shade (k_shade) apart from the framebuffer
                                 // writes (k_finish), which alone wait for earlier draws
+    int32_t depth_is_clear = 0; // binned mode: every framebuffer's depth plane holds
+                                // clear_depth (written just before, by k_clear /
+                                // k_clear_views): winners are tested against that
+                                // value instead of a load of the plane
     ViewDesc one;
 };
 

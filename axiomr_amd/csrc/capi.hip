@@ -904,9 +904,14 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     if (async && ka.mode == RASTER_VISIBILITY && !ctx->profiling && overlap_enabled())
         return draw_pipelined(ctx, fb, ka, fs);
     int r;
-    if ((fb->clear_pending && !ka.clear && (r = fb_settle(fb))) || (r = ctx_join(ctx)) ||
-        (r = arena_ensure(ctx, ctx->err, ctx->main, fs, ctx->need_floor)))
-        return r;
+    if (fb->clear_pending && !ka.clear) {
+        // binned mode: the clear is written now, so the winners are tested
+        // against its depth without loading the plane
+        if ((r = fb_settle(fb))) return r;
+        ka.depth_is_clear = 1;
+        ka.clear_depth = fb->clear_depth;
+    }
+    if ((r = ctx_join(ctx)) || (r = arena_ensure(ctx, ctx->err, ctx->main, fs, ctx->need_floor))) return r;
     for (int attempt = 0; attempt < 8; ++attempt) {
         if ((r = ctx_launch(ctx, ka))) return r;
         fb->clear_pending = false;   // written by this launch, even by a frame that overflows
@@ -1025,6 +1030,11 @@ int axr_batch_launch(axr_batch* b) {
     if (absorb) {
         ka.clear = 1;
         ka.clear_bgra = b->clear_bgra;
+        ka.clear_depth = b->clear_depth;
+    }
+    if (b->clear && !absorb) {
+        // k_clear_views writes the clear just before the frame's kernels
+        ka.depth_is_clear = 1;
         ka.clear_depth = b->clear_depth;
     }
     if (ctx->profiling) {   // events need the eager path

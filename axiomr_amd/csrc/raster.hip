@@ -942,8 +942,11 @@ __device__ __forceinline__ vec4 shade_winner(const ViewDesc& V, const Arena& ar,
     return shade_at<S>(V, ar, r.c.w, prim, clipped, slot, i0, i1, i2, bar);
 }
 
+// known_depth: the depth plane is known to hold *known_depth here (a clear
+// written just before the draw); null: load it.
 template <class S>
-__device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px, int py) {
+__device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px, int py,
+                                              const float* known_depth) {
     const uint32_t key = (uint32_t)k;
     const uint32_t rank = key >> 3, j = key & 7u;
     TriRec r = ar.rec[V.face_base + rank];
@@ -959,7 +962,7 @@ __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar
     float al, be, ga, z;
     cover(rt, px, py, al, be, ga, z);
     const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
-    if (!(z < V.depth[pix])) return;   // reference mergeTileResults: strict less
+    if (!(z < (known_depth ? *known_depth : V.depth[pix]))) return;   // reference mergeTileResults: strict less
     const vec4 col = shade_winner<S>(V, ar, r, slot, clipped, rank, make3(al, be, ga));
     V.depth[pix] = z;
     V.color[pix] = pack_bgra(col);
@@ -968,11 +971,11 @@ __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar
 
 template <int T, int NT, class S>
 __device__ __forceinline__ void resolve_tile(const ViewDesc& V, const Arena& ar, const unsigned long long* keys,
-                                             int ox, int oy) {
+                                             int ox, int oy, const float* known_depth) {
     for (int p = threadIdx.x; p < T * T; p += NT) {
         const unsigned long long k = keys[p];
         if (k == EMPTY_KEY) continue;
-        resolve_pixel<S>(V, ar, k, ox + (p % T), oy + (p / T));
+        resolve_pixel<S>(V, ar, k, ox + (p % T), oy + (p / T), known_depth);
     }
 }
 
@@ -987,17 +990,17 @@ template <> struct ShaderOf<SHADER_FLAT_COLOR> { using type = FlatColorShader; }
 
 template <int T, int NT, int SK>
 __device__ __forceinline__ void resolve_tile_kind(const ViewDesc& V, const Arena& ar, const unsigned long long* keys,
-                                                  int ox, int oy) {
+                                                  int ox, int oy, const float* kd) {
     if constexpr (SK >= 0) {
-        resolve_tile<T, NT, typename ShaderOf<SK>::type>(V, ar, keys, ox, oy);
+        resolve_tile<T, NT, typename ShaderOf<SK>::type>(V, ar, keys, ox, oy, kd);
     } else {
         switch (V.u.kind) {
-        case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy); break;
-        case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy); break;
-        case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy); break;
-        case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy); break;
-        case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy); break;
-        default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy); break;
+        case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy, kd); break;
+        case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy, kd); break;
+        case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy, kd); break;
+        case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy, kd); break;
+        case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy, kd); break;
+        default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy, kd); break;
         }
     }
 }
@@ -1066,7 +1069,7 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
         __syncthreads();
     }
 
-    resolve_tile_kind<T, NT, SK>(V, ar, keys, ox, oy);
+    resolve_tile_kind<T, NT, SK>(V, ar, keys, ox, oy, ka.depth_is_clear ? &ka.clear_depth : nullptr);
 }
 
 // One workgroup per entry of the list of non-empty bins (the grid covers
