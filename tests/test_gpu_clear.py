@@ -3,6 +3,10 @@ draw's last pass (visibility mode) or before any other use of the planes
 (read-back, binned draw, batch).  Whatever the order of calls, the pixels
 are those of the clear followed by the draws, as the CPU checker gives them."""
 import ctypes as C
+import os
+import subprocess
+import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -301,3 +305,31 @@ def _frames_without_ids(ctx, sc):
     for k in range(3):
         ref = oracle_sequence(sc, steps[k])
         H.assert_parity(H.compare(got[k], (ref[0], ref[1], None)), f"framebuffer {k}")
+
+
+_CHILD = """
+import sys
+sys.path[:0] = ["tests", "."]
+import test_gpu_clear as T
+import axiomr_amd as axr
+from axiomr_amd import _native as N
+from axiomr_amd import scenes as S
+ctx = axr.Context(0)
+ctx.set_config(tile_size=16, raster_mode=N.RASTER_VISIBILITY)
+T._frames_without_ids(ctx, S.icosphere_scene(n=16, width=200, height=150, tex=64))
+T.test_async_draws_keep_call_order(ctx)
+ctx.close()
+print("ok")
+"""
+
+
+@pytest.mark.parametrize("env", [{"AXR_STREAM_WAIT_VALUE": "0"}, {"AXR_OVERLAP": "0"}])
+def test_ordering_with_handoff_fallbacks(env):
+    """The ordering checks above in a process whose overlapped draws hand
+    the framebuffer over with events instead of stream memory waits, and in
+    one whose draws do not overlap (what a counter-collecting profiler
+    gets).  Both switches are read once per process, hence the child."""
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", _CHILD], cwd=root, env={**os.environ, **env}, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout[-3000:] + r.stderr[-3000:]
