@@ -174,7 +174,8 @@ class OracleRenderer:
     """CPU tiled pipeline with `threads` workers (reference ThreadPool)."""
 
     def __init__(self, threads: int = 0):
-        # the GPU box's CPU share is 16 threads whatever os.cpu_count() says
+        # at most 16 workers: the host share the benchmark machines give one
+        # GPU, whatever os.cpu_count() reports there
         self.threads = threads or min(16, os.cpu_count() or 1)
         self.pipe = lib().orc_pipeline_create(self.threads)
 
