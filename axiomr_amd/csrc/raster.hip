@@ -1233,11 +1233,12 @@ __device__ __forceinline__ void clear_pixel(const ViewDesc& V, size_t pix, const
     if (V.ids) V.ids[pix] = 0xFFFFFFFFu;
 }
 
+// Pixels p0, p0 + step, ... of the bin at (ox, oy), row-major.
 template <int T, class S>
 __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena& ar, int ox, int oy,
-                                                 const ClearSpec& cs) {
+                                                 const ClearSpec& cs, int p0, int step) {
     unsigned long long* vk = ar.vkeys + V.pix_base;
-    for (int p = (int)__lane_id(); p < T * T; p += 64) {
+    for (int p = p0; p < T * T; p += step) {
         const int px = ox + (p % T), py = oy + (p / T);
         if (px >= V.W || py >= V.H) continue;
         const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
@@ -1251,10 +1252,12 @@ __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena&
     }
 }
 
-// The pixels of a bin without keys, when the draw absorbs a clear.
+// Pixels p0, p0 + step, ... of a bin without keys, when the draw absorbs a
+// clear.
 template <int T>
-__device__ __forceinline__ void clear_vis_tile(const ViewDesc& V, int ox, int oy, const ClearSpec& cs) {
-    for (int p = (int)__lane_id(); p < T * T; p += 64) {
+__device__ __forceinline__ void clear_vis_tile(const ViewDesc& V, int ox, int oy, const ClearSpec& cs, int p0,
+                                               int step) {
+    for (int p = p0; p < T * T; p += step) {
         const int px = ox + (p % T), py = oy + (p / T);
         if (px >= V.W || py >= V.H) continue;
         clear_pixel(V, (size_t)py * (size_t)V.W + (size_t)px, cs);
@@ -1263,17 +1266,17 @@ __device__ __forceinline__ void clear_vis_tile(const ViewDesc& V, int ox, int oy
 
 template <int T, int SK>
 __device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const Arena& ar, int ox, int oy,
-                                                      const ClearSpec& cs) {
+                                                      const ClearSpec& cs, int p0, int step) {
     if constexpr (SK >= 0) {
-        resolve_vis_tile<T, typename ShaderOf<SK>::type>(V, ar, ox, oy, cs);
+        resolve_vis_tile<T, typename ShaderOf<SK>::type>(V, ar, ox, oy, cs, p0, step);
     } else {
         switch (V.u.kind) {
-        case SHADER_FLAT: resolve_vis_tile<T, FlatShader>(V, ar, ox, oy, cs); break;
-        case SHADER_PHONG: resolve_vis_tile<T, PhongShader>(V, ar, ox, oy, cs); break;
-        case SHADER_PBR: resolve_vis_tile<T, PBRShader>(V, ar, ox, oy, cs); break;
-        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader>(V, ar, ox, oy, cs); break;
-        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader>(V, ar, ox, oy, cs); break;
-        default: resolve_vis_tile<T, FlatColorShader>(V, ar, ox, oy, cs); break;
+        case SHADER_FLAT: resolve_vis_tile<T, FlatShader>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_PHONG: resolve_vis_tile<T, PhongShader>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_PBR: resolve_vis_tile<T, PBRShader>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader>(V, ar, ox, oy, cs, p0, step); break;
+        default: resolve_vis_tile<T, FlatColorShader>(V, ar, ox, oy, cs, p0, step); break;
         }
     }
 }
@@ -1302,20 +1305,40 @@ __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
         ar.counters[CNT_CLIP_RECS] = 0;
         *big_counter(ar) = 0ull;
     }
-    const uint32_t nw = gridDim.x * 4u;
-    for (uint32_t t = (blockIdx.x * 256u + threadIdx.x) >> 6; t < ka.total_tiles; t += nw) {
-        // the same word for the whole wave; the branch waits for the load, so
-        // every lane has read the flag before lane 0 clears it
-        const bool keys = !failed && ar.tile_count[t] != 0u;
-        if (!keys && !cs.on) continue;
-        if (keys && __lane_id() == 0) ar.tile_count[t] = 0u;
-        uint32_t vi = 0;
-        if constexpr (BATCH) vi = ka.tile_views[t];
-        const ViewDesc& V = view_at<BATCH>(ka, vi);
-        const uint32_t local = t - V.tile_base;
-        const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
-        if (keys) resolve_vis_tile_kind<T, SK>(V, ar, ox, oy, cs);
-        else clear_vis_tile<T>(V, ox, oy, cs);
+    if constexpr (T == 8) {
+        // one wave per bin
+        const uint32_t nw = gridDim.x * 4u;
+        for (uint32_t t = (blockIdx.x * 256u + threadIdx.x) >> 6; t < ka.total_tiles; t += nw) {
+            // the same word for the whole wave; the branch waits for the
+            // load, so every lane has read the flag before lane 0 clears it
+            const bool keys = !failed && ar.tile_count[t] != 0u;
+            if (!keys && !cs.on) continue;
+            if (keys && __lane_id() == 0) ar.tile_count[t] = 0u;
+            uint32_t vi = 0;
+            if constexpr (BATCH) vi = ka.tile_views[t];
+            const ViewDesc& V = view_at<BATCH>(ka, vi);
+            const uint32_t local = t - V.tile_base;
+            const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
+            if (keys) resolve_vis_tile_kind<T, SK>(V, ar, ox, oy, cs, (int)__lane_id(), 64);
+            else clear_vis_tile<T>(V, ox, oy, cs, (int)__lane_id(), 64);
+        }
+    } else {
+        // one workgroup (four waves, a pixel or more per thread) per bin:
+        // every wave reads the flag before the barrier, thread 0 clears it
+        // after
+        for (uint32_t t = blockIdx.x; t < ka.total_tiles; t += gridDim.x) {
+            const bool keys = !failed && ar.tile_count[t] != 0u;
+            __syncthreads();
+            if (keys && threadIdx.x == 0) ar.tile_count[t] = 0u;
+            if (!keys && !cs.on) continue;
+            uint32_t vi = 0;
+            if constexpr (BATCH) vi = ka.tile_views[t];
+            const ViewDesc& V = view_at<BATCH>(ka, vi);
+            const uint32_t local = t - V.tile_base;
+            const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
+            if (keys) resolve_vis_tile_kind<T, SK>(V, ar, ox, oy, cs, (int)threadIdx.x, 256);
+            else clear_vis_tile<T>(V, ox, oy, cs, (int)threadIdx.x, 256);
+        }
     }
 }
 
@@ -1545,7 +1568,6 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     if (ev) (void)hipEventRecord(ev[1], st);
     k_big<T, BATCH><<<1024, 256, 0, st>>>(ka, ar);
     if (ev) (void)hipEventRecord(ev[2], st);
-    const int rb = imax(1, imin((int)((ka.total_tiles + 3) / 4), RESOLVE_MAX_BLOCKS));
     if (!BATCH && ka.two_pass && ka.clear && !ka.one.ids && ka.shader >= 0 && ka.shader < SHADER_COUNT) {
         // shade at once, write the framebuffer after the earlier draws
         const uint64_t units = (uint64_t)ka.total_tiles * (uint64_t)((T * T + 63) / 64);
@@ -1575,6 +1597,9 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     // the framebuffer pass waits for what other streams were asked to do first
     hipError_t e = fb_wait(st, wait);
     if (e != hipSuccess) return e;
+    // T = 8: four bins per workgroup; larger bins: one bin per workgroup
+    const uint32_t per_block = T == 8 ? 4u : 1u;
+    const int rb = imax(1, imin((int)((ka.total_tiles + per_block - 1) / per_block), RESOLVE_MAX_BLOCKS));
     switch (ka.shader) {
     case SHADER_FLAT: k_resolve<T, BATCH, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
     case SHADER_PHONG: k_resolve<T, BATCH, SHADER_PHONG><<<rb, 256, 0, st>>>(ka, ar); break;
