@@ -369,9 +369,10 @@ static bool overlap_enabled() {
 int ctx_join(axr_ctx* ctx) {
     hipError_t e = hipSuccess;
     for (int k = 1; k < axr_ctx::kArenas && e == hipSuccess; ++k) {
-        if (!ctx->alt[k - 1].stream) continue;
+        if (!ctx->alt[k - 1].stream || !ctx->alt_unjoined[k - 1]) continue;
         e = hipEventRecord(ctx->join_ev[k], ctx->alt[k - 1].stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(ctx->main.stream, ctx->join_ev[k], 0);
+        if (e == hipSuccess) ctx->alt_unjoined[k - 1] = false;
     }
     if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
     ctx->last_arena = 0;
@@ -853,6 +854,7 @@ static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const Frame
             if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
         }
     }
+    if (x) ctx->alt_unjoined[x - 1] = true;   // before the launch: a failed one may have queued work
     if ((r = arena_launch(ctx->err, fa, ka, nullptr, wait))) return r;
     fb->clear_pending = false;   // written by this launch, even by a frame that overflows
     // the end of this draw's framebuffer pass, for the next draw to wait on

@@ -83,6 +83,7 @@ struct axr_ctx {
     hipEvent_t join_ev[kArenas] = {};   // join_ev[k]: recorded on arena k's stream for another to wait on
     int next_arena = 0;                 // arena of the next asynchronous visibility-mode draw
     int last_arena = 0;                 // arena whose stream took the latest framebuffer work
+    bool alt_unjoined[kArenas - 1] = {};  // alt[k] took work main has not waited for
     axr::ArenaNeeds need_floor;         // the largest overflow needs reported: every arena is grown to them
     uint64_t last_value = 0;            // nonzero: that work was a draw whose stream then wrote this
                                         // value to its arena's signal (0: wait with an event)
