@@ -858,6 +858,9 @@ static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const Frame
     if ((r = arena_launch(ctx->err, fa, ka, nullptr, wait))) return r;
     fb->clear_pending = false;   // written by this launch, even by a frame that overflows
     // the end of this draw's framebuffer pass, for the next draw to wait on
+    // (with last_value 0 the next draw waits with an event on this stream)
+    ctx->last_arena = x;
+    ctx->next_arena = (x + 1) % axr_ctx::kArenas;
     ctx->last_value = 0;
     if (fa.sig) {
         const uint64_t v = ++g_sig_value;
@@ -865,8 +868,6 @@ static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const Frame
         if (e != hipSuccess) return hip_fail(ctx->err, e, "stream order");
         ctx->last_value = v;
     }
-    ctx->last_arena = x;
-    ctx->next_arena = (x + 1) % axr_ctx::kArenas;
     return AXR_OK;
 }
 
