@@ -942,11 +942,11 @@ __device__ __forceinline__ vec4 shade_winner(const ViewDesc& V, const Arena& ar,
     return shade_at<S>(V, ar, r.c.w, prim, clipped, slot, i0, i1, i2, bar);
 }
 
-// known_depth: the depth plane is known to hold *known_depth here (a clear
-// written just before the draw); null: load it.
+// known: the depth plane is known to hold known_depth here (a clear written
+// just before the draw); else it is loaded.
 template <class S>
 __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px, int py,
-                                              const float* known_depth) {
+                                              bool known, float known_depth) {
     const uint32_t key = (uint32_t)k;
     const uint32_t rank = key >> 3, j = key & 7u;
     TriRec r = ar.rec[V.face_base + rank];
@@ -962,7 +962,7 @@ __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar
     float al, be, ga, z;
     cover(rt, px, py, al, be, ga, z);
     const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
-    if (!(z < (known_depth ? *known_depth : V.depth[pix]))) return;   // reference mergeTileResults: strict less
+    if (!(z < (known ? known_depth : V.depth[pix]))) return;   // reference mergeTileResults: strict less
     const vec4 col = shade_winner<S>(V, ar, r, slot, clipped, rank, make3(al, be, ga));
     V.depth[pix] = z;
     V.color[pix] = pack_bgra(col);
@@ -971,11 +971,11 @@ __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar
 
 template <int T, int NT, class S>
 __device__ __forceinline__ void resolve_tile(const ViewDesc& V, const Arena& ar, const unsigned long long* keys,
-                                             int ox, int oy, const float* known_depth) {
+                                             int ox, int oy, bool known, float known_depth) {
     for (int p = threadIdx.x; p < T * T; p += NT) {
         const unsigned long long k = keys[p];
         if (k == EMPTY_KEY) continue;
-        resolve_pixel<S>(V, ar, k, ox + (p % T), oy + (p / T), known_depth);
+        resolve_pixel<S>(V, ar, k, ox + (p % T), oy + (p / T), known, known_depth);
     }
 }
 
@@ -990,17 +990,17 @@ template <> struct ShaderOf<SHADER_FLAT_COLOR> { using type = FlatColorShader; }
 
 template <int T, int NT, int SK>
 __device__ __forceinline__ void resolve_tile_kind(const ViewDesc& V, const Arena& ar, const unsigned long long* keys,
-                                                  int ox, int oy, const float* kd) {
+                                                  int ox, int oy, bool kn, float kd) {
     if constexpr (SK >= 0) {
-        resolve_tile<T, NT, typename ShaderOf<SK>::type>(V, ar, keys, ox, oy, kd);
+        resolve_tile<T, NT, typename ShaderOf<SK>::type>(V, ar, keys, ox, oy, kn, kd);
     } else {
         switch (V.u.kind) {
-        case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy, kd); break;
-        case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy, kd); break;
-        case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy, kd); break;
-        case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy, kd); break;
-        case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy, kd); break;
-        default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy, kd); break;
+        case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy, kn, kd); break;
+        case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy, kn, kd); break;
+        case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy, kn, kd); break;
+        case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy, kn, kd); break;
+        case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy, kn, kd); break;
+        default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy, kn, kd); break;
         }
     }
 }
@@ -1069,7 +1069,7 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
         __syncthreads();
     }
 
-    resolve_tile_kind<T, NT, SK>(V, ar, keys, ox, oy, ka.depth_is_clear ? &ka.clear_depth : nullptr);
+    resolve_tile_kind<T, NT, SK>(V, ar, keys, ox, oy, ka.depth_is_clear != 0, ka.clear_depth);
 }
 
 // One workgroup per entry of the list of non-empty bins (the grid covers
@@ -1080,8 +1080,16 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
 // (see k_resolve).
 constexpr uint32_t NUM_XCD = 8;
 
+// Six waves per SIMD (at most 80 VGPRs) for the shader kinds whose raster
+// kernel fits that without spilling: the batch's k_raster, ALU-bound, hides
+// more of its load latency with the extra wave (2.49 -> 2.43 ms).  Phong,
+// PBR and the mixed kernel would spill and keep the default.
+constexpr int raster_min_waves(int sk) {
+    return (sk == SHADER_FLAT || sk == SHADER_GOURAUD || sk == SHADER_PHONG_POINT || sk == SHADER_FLAT_COLOR) ? 6 : 1;
+}
+
 template <int T, int NT, bool BATCH, int SK>
-__global__ __launch_bounds__(NT) void k_raster(KArgs ka, Arena ar) {
+__global__ __launch_bounds__(NT, raster_min_waves(SK)) void k_raster(KArgs ka, Arena ar) {
     __shared__ unsigned long long keys[T * T];
     __shared__ uint32_t bigq[NT];
     __shared__ uint32_t nbig;
@@ -1411,22 +1419,26 @@ __global__ __launch_bounds__(256) void k_finish(KArgs ka, Arena ar) {
         const bool keys = !failed && in_x && ar.tile_count[t] != 0u;
         const int rows = imin(T, V.H - (int)by * T);
         const size_t pix0 = (size_t)by * T * (size_t)V.W + (size_t)px;
-        // all slot loads first: the stores below may alias them for the
-        // compiler, which would otherwise wait for each row's load in turn
-        unsigned long long sl[T];
+        constexpr int RB = T < 8 ? T : 8;   // rows whose slots are loaded together
+        for (int r0 = 0; r0 < rows; r0 += RB) {
+            // all slot loads of RB rows first: the stores below may alias
+            // them for the compiler, which would otherwise wait for each
+            unsigned long long sl[RB];
 #pragma unroll
-        for (int r = 0; r < T; ++r) sl[r] = keys && r < rows ? vk[pix0 + (size_t)r * V.W] : EMPTY_KEY;
-        if (!in_x) continue;
+            for (int r = 0; r < RB; ++r)
+                sl[r] = keys && r0 + r < rows ? vk[pix0 + (size_t)(r0 + r) * V.W] : EMPTY_KEY;
+            if (!in_x) continue;
 #pragma unroll
-        for (int r = 0; r < T; ++r) {
-            if (r >= rows) break;
-            const size_t pix = pix0 + (size_t)r * V.W;
-            if (sl[r] != EMPTY_KEY) {
-                vk[pix] = EMPTY_KEY;
-                V.color[pix] = (uint32_t)sl[r];
-                V.depth[pix] = __uint_as_float((uint32_t)(sl[r] >> 32));
-            } else {
-                clear_pixel(V, pix, cs);
+            for (int r = 0; r < RB; ++r) {
+                if (r0 + r >= rows) break;
+                const size_t pix = pix0 + (size_t)(r0 + r) * V.W;
+                if (sl[r] != EMPTY_KEY) {
+                    vk[pix] = EMPTY_KEY;
+                    V.color[pix] = (uint32_t)sl[r];
+                    V.depth[pix] = __uint_as_float((uint32_t)(sl[r] >> 32));
+                } else {
+                    clear_pixel(V, pix, cs);
+                }
             }
         }
         // every lane of a bin read its flag before the loop's loads returned
