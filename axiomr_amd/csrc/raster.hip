@@ -875,10 +875,15 @@ __global__ __launch_bounds__(256) void k_scatter(Arena ar, uint32_t nwaves) {
     if (ar.counters[CNT_ERROR]) return;
     const uint32_t stride = gridDim.x * 256u;
     const uint32_t cap = ar.wave_cap;
-    const uint32_t nreg = nwaves * cap;
-    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < nreg; q += stride) {
-        if ((q % cap) < ar.wave_pairs[q / cap]) {
-            const uint4 p = ar.pairs[q];
+    // one wave per setup wave's region: its pair count is read once and
+    // only the pairs it holds are visited
+    const uint32_t lane = __lane_id();
+    const uint32_t nw = gridDim.x * 4u;
+    for (uint32_t w = (blockIdx.x * 256u + threadIdx.x) >> 6; w < nwaves; w += nw) {
+        const uint32_t n = ar.wave_pairs[w];
+        const uint4* reg = ar.pairs + (size_t)w * cap;
+        for (uint32_t i = lane; i < n; i += 64u) {
+            const uint4 p = reg[i];
             ar.bins[ar.tile_off[p.x] + p.y] = p.z;
         }
     }
@@ -1645,7 +1650,7 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
     }
     if (ev) (void)hipEventRecord(ev[2], st);
     const uint32_t nwaves = setup_blocks * 4;
-    const uint32_t sblocks = imax(64, imin(2048, (int)((nwaves * ar.wave_cap / 8 + 255) / 256)));
+    const uint32_t sblocks = imax(64, imin(8192, (int)((nwaves + 3) / 4)));
     k_scatter<<<sblocks, 256, 0, st>>>(ar, nwaves);
     if (ev) (void)hipEventRecord(ev[3], st);
     const uint32_t rblocks = ((imax(1, (int)ntiles) + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
