@@ -214,9 +214,10 @@ static int arena_reset(ErrSink& err, FrameArena& fa) {
 // written with hipStreamWriteValue64.  Signals are never freed (the runtime
 // serves them from memory the host and the command processor share);
 // an arena that is destroyed returns its signal to a per-device free list
-// for the next one.  Every value written comes from one process-wide
-// counter, so a signal's value only grows and a wait for "at least v" is
-// met by the write of v and by nothing written before it.
+// for the next one.  An arena zeroes the signal it takes (nothing waits on
+// it then); the values it writes afterwards come from one process-wide
+// counter, so they only grow and a wait for "at least v" is met by the
+// write of v and by nothing written before it.
 namespace {
 std::mutex g_sig_mu;
 std::vector<std::pair<int, uint64_t*>> g_sig_free;   // (device, signal)
