@@ -307,6 +307,46 @@ def _frames_without_ids(ctx, sc):
         H.assert_parity(H.compare(got[k], (ref[0], ref[1], None)), f"framebuffer {k}")
 
 
+@pytest.mark.parametrize("tile", [8, 16])
+def test_async_soup_with_clipping_and_ties_two_pass(tile):
+    """Shading ahead of the framebuffer pass on a scene with clipped faces
+    (their pieces read back through the clip records) and exactly equal
+    depths (submission order decides): queued clear-then-draw frames into a
+    framebuffer without an id plane, two of them into the same framebuffer
+    back to back.  A fresh context's arenas are too small for this scene's
+    large triangles: the first sync reports the overflow, every arena is
+    grown to it, and the same frames drawn again succeed."""
+    sc = S.soup_scene(ntri=60_000, width=480, height=270, seed=9)
+    ref = H.render_oracle(sc)
+    ctx = R.Context(0)
+    try:
+        ctx.set_config(tile_size=tile, raster_mode=N.RASTER_VISIBILITY)
+        gs = H.GpuScene(ctx, sc)
+        fbs = [R.Framebuffer(ctx, sc.width, sc.height) for _ in range(2)]
+        try:
+            p = H.params_of(sc, flags=N.AXR_DRAW_ASYNC)
+            overflows = 0
+            while True:
+                for fb in (fbs[0], fbs[1], fbs[0]):
+                    fb.clear(H.BLACK)
+                    draw(ctx, fb, gs, p)
+                if ctx._lib.axr_sync(ctx.handle) != N.AXR_ERR_OVERFLOW:
+                    break
+                overflows += 1
+                assert overflows < 2, "arenas not grown to the reported needs"
+            ctx.check(ctx._lib.axr_sync(ctx.handle))
+            got = [fb.read() for fb in fbs]
+        finally:
+            for fb in fbs:
+                fb.close()
+            gs.close()
+    finally:
+        ctx.close()
+    for k, g in enumerate(got):
+        rep = H.compare(g, (ref[0], ref[1], None))
+        H.assert_parity(rep, f"soup tile {tile} framebuffer {k}")
+        assert rep["covered"] > 50_000
+
 _CHILD = """
 import sys
 sys.path[:0] = ["tests", "."]
