@@ -975,12 +975,23 @@ __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar
 }
 
 template <int T, int NT, class S>
+// The bin's covered pixels are listed first (plist, count in *ncov, which
+// starts at zero), so that every wave shades covered pixels only: a bin on
+// a silhouette takes fewer shading rounds than one pass over its rows.
 __device__ __forceinline__ void resolve_tile(const ViewDesc& V, const Arena& ar, const unsigned long long* keys,
-                                             int ox, int oy, bool known, float known_depth) {
+                                             int ox, int oy, bool known, float known_depth, uint16_t* plist,
+                                             uint32_t* ncov) {
+    static_assert((T * T) % 64 == 0 && NT % 64 == 0, "every wave makes the same number of passes");
     for (int p = threadIdx.x; p < T * T; p += NT) {
-        const unsigned long long k = keys[p];
-        if (k == EMPTY_KEY) continue;
-        resolve_pixel<S>(V, ar, k, ox + (p % T), oy + (p / T), known, known_depth);
+        const bool cov = keys[p] != EMPTY_KEY;
+        const uint32_t slot = wave_append(ncov, cov);
+        if (cov) plist[slot] = (uint16_t)p;
+    }
+    __syncthreads();
+    const uint32_t n = *ncov;
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+        const int p = plist[i];
+        resolve_pixel<S>(V, ar, keys[p], ox + (p % T), oy + (p / T), known, known_depth);
     }
 }
 
@@ -995,24 +1006,24 @@ template <> struct ShaderOf<SHADER_FLAT_COLOR> { using type = FlatColorShader; }
 
 template <int T, int NT, int SK>
 __device__ __forceinline__ void resolve_tile_kind(const ViewDesc& V, const Arena& ar, const unsigned long long* keys,
-                                                  int ox, int oy, bool kn, float kd) {
+                                                  int ox, int oy, bool kn, float kd, uint16_t* pl, uint32_t* nc) {
     if constexpr (SK >= 0) {
-        resolve_tile<T, NT, typename ShaderOf<SK>::type>(V, ar, keys, ox, oy, kn, kd);
+        resolve_tile<T, NT, typename ShaderOf<SK>::type>(V, ar, keys, ox, oy, kn, kd, pl, nc);
     } else {
         switch (V.u.kind) {
-        case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy, kn, kd); break;
-        case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy, kn, kd); break;
-        case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy, kn, kd); break;
-        case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy, kn, kd); break;
-        case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy, kn, kd); break;
-        default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy, kn, kd); break;
+        case SHADER_FLAT: resolve_tile<T, NT, FlatShader>(V, ar, keys, ox, oy, kn, kd, pl, nc); break;
+        case SHADER_PHONG: resolve_tile<T, NT, PhongShader>(V, ar, keys, ox, oy, kn, kd, pl, nc); break;
+        case SHADER_PBR: resolve_tile<T, NT, PBRShader>(V, ar, keys, ox, oy, kn, kd, pl, nc); break;
+        case SHADER_GOURAUD: resolve_tile<T, NT, GouraudShader>(V, ar, keys, ox, oy, kn, kd, pl, nc); break;
+        case SHADER_PHONG_POINT: resolve_tile<T, NT, PhongPointShader>(V, ar, keys, ox, oy, kn, kd, pl, nc); break;
+        default: resolve_tile<T, NT, FlatColorShader>(V, ar, keys, ox, oy, kn, kd, pl, nc); break;
         }
     }
 }
 
 template <int T, int NT, bool BATCH, int SK>
 __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uint32_t t, unsigned long long* keys,
-                                           uint32_t* bigq, uint32_t& nbig) {
+                                           uint32_t* bigq, uint32_t& nbig, uint16_t* plist) {
     uint32_t vi = 0;
     if constexpr (BATCH) vi = ka.tile_views[t];
     const ViewDesc& V = view_at<BATCH>(ka, vi);
@@ -1074,7 +1085,8 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
         __syncthreads();
     }
 
-    resolve_tile_kind<T, NT, SK>(V, ar, keys, ox, oy, ka.depth_is_clear != 0, ka.clear_depth);
+    // nbig is zero again here: it counts the covered pixels
+    resolve_tile_kind<T, NT, SK>(V, ar, keys, ox, oy, ka.depth_is_clear != 0, ka.clear_depth, plist, &nbig);
 }
 
 // One workgroup per entry of the list of non-empty bins (the grid covers
@@ -1097,6 +1109,7 @@ template <int T, int NT, bool BATCH, int SK>
 __global__ __launch_bounds__(NT, raster_min_waves(SK)) void k_raster(KArgs ka, Arena ar) {
     __shared__ unsigned long long keys[T * T];
     __shared__ uint32_t bigq[NT];
+    __shared__ uint16_t plist[T * T];
     __shared__ uint32_t nbig;
     if (ar.counters[CNT_ERROR]) return;
     // the pair overflow count was last needed by the scatter
@@ -1106,7 +1119,7 @@ __global__ __launch_bounds__(NT, raster_min_waves(SK)) void k_raster(KArgs ka, A
     const uint32_t j = blockIdx.x / NUM_XCD;
     const uint32_t a = (blockIdx.x % NUM_XCD) * chunk + j;
     if (j >= chunk || a >= nactive) return;
-    raster_bin<T, NT, BATCH, SK>(ka, ar, ar.active[a], keys, bigq, nbig);
+    raster_bin<T, NT, BATCH, SK>(ka, ar, ar.active[a], keys, bigq, nbig, plist);
 }
 
 // ---------------------------------------------------------------------------
