@@ -105,3 +105,28 @@ def resolve(sx, sy, z, keys, W, H):
         depth = np.where(better, zz, depth)
         ids = np.where(better, np.uint32(keys[t]), ids)
     return depth.astype(f32), ids
+
+
+def merge_draws(draws, W, H, clear_bgra, clear_depth):
+    """Framebuffer after a clear and a list of draws, each draw given as
+    (sx, sy, z, keys, bgra per triangle): the draw's own winners
+    (`resolve`, from +inf as the reference's tile buffers start), then the
+    reference's merge (tiled_pipeline.cpp:1125-1181): a pixel takes the
+    draw's depth and colour only if it is strictly less than what the
+    framebuffer holds, so an earlier draw or the clear keeps every tie.
+    Returns colour (H, W) uint32 BGRA-packed, depth (H, W) float32, and the
+    (draw index, key) of each written pixel (-1, EMPTY where the clear
+    stayed)."""
+    color = np.full((H, W), np.uint32(clear_bgra), np.uint32)
+    depth = np.full((H, W), f32(clear_depth), f32)
+    src = np.full((H, W), -1, np.int64)
+    ids = np.full((H, W), EMPTY, np.uint32)
+    for d, (sx, sy, z, keys, bgra) in enumerate(draws):
+        dz, did = resolve(sx, sy, z, keys, W, H)
+        win = (did != EMPTY) & (dz < depth)
+        slot = np.searchsorted(keys, did[win])
+        color[win] = np.asarray(bgra, np.uint32)[slot]
+        depth[win] = dz[win]
+        src[win] = d
+        ids[win] = did[win]
+    return color, depth, src, ids

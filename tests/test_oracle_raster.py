@@ -153,3 +153,69 @@ def test_pixel_centres_on_edges_and_corners():
         [ndc(15, 15) + [0.1], ndc(17, 15) + [0.1], ndc(16, 17) + [0.1]],
     ], f32)
     assert check(ndc_scene(tris, W, Hh), tris) > 300
+
+
+def _colored(tris, colors):
+    return ndc_scene(tris, 64, 48, colors=np.asarray(colors, np.uint32))
+
+
+def _brute_draw(tris, colors, W, Hh):
+    clip = transform(np.eye(4, dtype=f32).reshape(16), tris.astype(f32))
+    sx, sy, z = RC.screen_setup(clip, W, Hh)
+    keep = ~RC.backface(sx, sy)
+    keys = np.arange(len(tris), dtype=np.uint32) * 8
+    # flat colours are written back exactly, as BGRA
+    c = np.asarray(colors, np.uint32)
+    bgra = ((c >> 16) & 255) | (c & 0xFF00FF00) | ((c & 255) << 16)
+    return sx[keep], sy[keep], z[keep], keys[keep], bgra[keep]
+
+
+def _oracle_draws(scenes, W, Hh, clear_depth):
+    import oracle as O
+    rr = O.OracleRenderer(2)
+    fb = rr.framebuffer(W, Hh)
+    oscs = [O.OracleScene(s) for s in scenes]
+    try:
+        O.fb_clear(fb, 0xFF000000, clear_depth)
+        for o in oscs:
+            rr.draw(fb, o, identity_params())
+        return O.fb_read(fb, W, Hh)
+    finally:
+        for o in oscs:
+            o.close()
+        O.fb_destroy(fb)
+        rr.close()
+
+
+@pytest.mark.parametrize("clear_at", [None, "winner"])
+def test_merge_across_draws_first_wins(clear_at):
+    """Two draws of the same triangles with different flat colours: every
+    depth tie keeps the first draw's colour (reference tiled_pipeline.cpp:
+    1150, strict less); a clear depth equal to a winner's exact depth keeps
+    the clear there.  The checker's framebuffer is compared with the
+    per-pixel brute force of rastercheck.merge_draws."""
+    W, Hh = 64, 48
+    rng = np.random.default_rng(21)
+    tris = random_tris(rng, 30, 0.3)
+    ca = rng.integers(0, 2**32, 30, dtype=np.uint64).astype(np.uint32)
+    cb = rng.integers(0, 2**32, 30, dtype=np.uint64).astype(np.uint32)
+    clear_depth = np.inf
+    if clear_at == "winner":
+        # the exact depth of a covered pixel of the first draw
+        _, d0, _ = _oracle_draws([_colored(tris, ca)], W, Hh, np.inf)
+        fin = np.sort(d0[np.isfinite(d0)])
+        clear_depth = float(fin[len(fin) // 2])
+    draws = [_brute_draw(tris, ca, W, Hh), _brute_draw(tris, cb, W, Hh)]
+    col, depth, src, ids = RC.merge_draws(draws, W, Hh, 0xFF000000, clear_depth)
+    bgra, odepth, oids = _oracle_draws([_colored(tris, ca), _colored(tris, cb)], W, Hh, clear_depth)
+    assert (odepth.view(np.uint32) == depth.view(np.uint32)).all(), "depth bits differ"
+    assert (bgra.view(np.uint32).reshape(Hh, W) == col).all(), "colours differ"
+    # the second draw ties everywhere it covers, so it never writes
+    assert (src != 1).all()
+    assert (src == 0).sum() > 200
+    if clear_at == "winner":
+        # the pixels whose winner sits exactly at the clear depth keep the clear
+        _, d0, _ = _oracle_draws([_colored(tris, ca)], W, Hh, np.inf)
+        at = d0 == np.float32(clear_depth)
+        assert at.any() and (src[at] == -1).all() and (col[at] == 0xFF000000).all()
+        assert ((d0 > np.float32(clear_depth)) & np.isfinite(d0)).any()
