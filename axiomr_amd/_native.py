@@ -68,7 +68,7 @@ class AxrKernelStats(C.Structure):
 class AxrServerConfig(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("max_batch_views", C.c_uint32), ("max_batch_faces", C.c_uint64),
                 ("max_batch_tiles", C.c_uint64), ("hbm_budget", C.c_uint64), ("clear_bgra", C.c_uint32),
-                ("with_ids", C.c_int32)]
+                ("with_ids", C.c_int32), ("max_batch_pixels", C.c_uint64)]
 
 
 class AxrServeRequest(C.Structure):
@@ -146,6 +146,7 @@ SIGNATURES = {
     "axr_fb_clear": (C.c_int, [_P, C.c_uint32, C.c_float]),
     "axr_fb_read": (C.c_int, [_P, _U8, _F, _U32]),
     "axr_draw": (C.c_int, [_P, _P, _P, C.POINTER(AxrDrawParams)]),
+    "axr_check_view": (C.c_int, [C.c_uint64, C.c_int, C.c_int]),
     "axr_batch_create": (C.c_int, [_P, C.c_uint32, _PP, _PP, C.POINTER(AxrDrawParams), _PP]),
     "axr_batch_set_clear": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_float]),
     "axr_batch_launch": (C.c_int, [_P]),

@@ -116,6 +116,15 @@ constexpr uint32_t REC_CLIPPED = 1u << 31;
 constexpr uint32_t REC_MATERIAL_MASK = 0xFFFFu;
 constexpr uint32_t REF_CLIP = 1u << 31;     // a bin entry with this bit names a clip record
 
+// Size limits of one view (one drawMesh).  The submission key is
+// rank * 8 + clip sub-triangle in 32 bits, so a view submits fewer than
+// 2^29 faces; framebuffer edges are at most 32768 pixels.  Face slots of a
+// whole batch are indexed in 31 bits (the top bit of a bin entry flags a
+// clip record).
+constexpr uint64_t MAX_VIEW_FACES = 1ull << 29;
+constexpr int MAX_FB_EDGE = 32768;
+constexpr uint64_t MAX_BATCH_FACES = 0x7FFFFFFFull;
+
 // Counter block at the head of the frame arena.
 enum Counter : uint32_t {
     CNT_CLIP_FACES = 0,   // faces queued for the full clipper
@@ -148,7 +157,9 @@ enum RasterMode : int32_t {
 // Device frame arena: the per-frame working memory of the binning pipeline.
 struct Arena {
     uint32_t* counters;        // CNT_COUNT words
-    TriRec* rec;               // one slot per face of every view
+    TriRec* rec;               // binned mode: one record per face slot of every view
+    uint32_t* clip_base;       // visibility mode (same memory as rec): first clip
+                               // record of each clipped face slot
     uint32_t* clip_faces;      // global face slots queued for clipping
     TriRec* clip_rec;          // clipped sub-triangles
     Vertex56* clip_verts;      // 3 per clip record
@@ -171,6 +182,8 @@ struct Arena {
     // visibility mode
     unsigned long long* vkeys; // per pixel of every view: (ordered depth << 32 | key),
                                // all ones when empty; the resolve leaves it empty
+    uint32_t* vcol;            // two-pass resolve into a framebuffer with ids: each
+                               // winner's colour, per pixel of every view
     TriRec* big_rec;           // large triangles (and clipped pieces) as records
     uint32_t* big_start;       // first work item of each large-triangle record
     uint32_t cap_vkeys;
@@ -201,9 +214,9 @@ struct KArgs {
     int32_t clear = 0;          // visibility mode: k_resolve also clears every view's
     uint32_t clear_bgra = 0;    // framebuffer to these values (a clear the draw
     float clear_depth = 0.0f;   // absorbs instead of a pass of its own)
-    int32_t two_pass = 0;       // plain visibility-mode draw with a clear and no id
-                                // plane: shade (k_shade) apart from the framebuffer
-                                // writes (k_finish), which alone wait for earlier draws
+    int32_t two_pass = 0;       // plain visibility-mode draw with a clear: shade
+                                // (k_shade) apart from the framebuffer writes
+                                // (k_finish), which alone wait for earlier draws
     int32_t depth_is_clear = 0; // binned mode: every framebuffer's depth plane holds
                                 // clear_depth (written just before, by k_clear /
                                 // k_clear_views): winners are tested against that

@@ -16,6 +16,8 @@ struct axr_batch {
     uint32_t nviews;
     std::vector<ViewDesc> views;
     std::vector<axr_fb*> fbs;      // the views' framebuffers (for their pending clears)
+    std::vector<const axr_mesh*> meshes;
+    bool holds_refs = false;       // fbs and meshes count this batch in batch_refs
     ViewDesc* d_views = nullptr;
     uint32_t* d_face_bases = nullptr;
     uint32_t* d_tile_bases = nullptr;
@@ -134,12 +136,16 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs
         return fail(err, AXR_ERR_INVALID, "draw too large (%llu faces, %llu bins, %llu pixels)",
                     (unsigned long long)faces, (unsigned long long)tiles, (unsigned long long)fs.pixels);
     Arena& a = fa.a;
+    // visibility mode (fs.pixels != 0) keeps no triangle records per face
+    // slot (only the first clip record of a clipped face, 4 bytes) and no
+    // bin pairs; binned mode keeps a 48-byte record per face slot
+    const bool vis = fs.pixels != 0;
     uint64_t clip_faces = std::max<uint64_t>({4096, faces / 16, need.clip_faces, a.cap_clip_faces});
     uint64_t clip_recs = std::max<uint64_t>({2 * clip_faces, need.clip_recs, a.cap_clip_recs});
     // one pair region per setup wave (setup runs 4 waves per 256 faces)
-    uint64_t waves = std::max<uint64_t>(4 * ((faces + 255) / 256), a.cap_waves);
-    uint64_t ovf = std::max<uint64_t>({faces / 4 + 16 * tiles + 4096, need.pairs, a.cap_ovf});
-    uint64_t nbins = std::max<uint64_t>({2 * faces + 16 * tiles + 4096, need.pairs, a.cap_bins});
+    uint64_t waves = vis ? a.cap_waves : std::max<uint64_t>(4 * ((faces + 255) / 256), a.cap_waves);
+    uint64_t ovf = vis ? a.cap_ovf : std::max<uint64_t>({faces / 4 + 16 * tiles + 4096, need.pairs, a.cap_ovf});
+    uint64_t nbins = vis ? a.cap_bins : std::max<uint64_t>({2 * faces + 16 * tiles + 4096, need.pairs, a.cap_bins});
     uint64_t scan_blocks = (tiles + 4095) / 4096 + 1;
     // visibility mode: large triangles and their 256-pixel work items
     uint64_t big_recs = 0, big_items = 0;
@@ -150,7 +156,7 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs
     const uint64_t gen_start = fa.gen;
     int r;
     if ((r = grow(ctx, err, fa, fa.counters, CNT_COUNT * 4)) ||
-        (r = grow(ctx, err, fa, fa.rec, std::max<uint64_t>(faces, 1) * sizeof(TriRec))) ||
+        (r = grow(ctx, err, fa, fa.rec, std::max<uint64_t>(faces, 1) * (vis ? 4 : sizeof(TriRec)))) ||
         (r = grow(ctx, err, fa, fa.clip_faces, clip_faces * 4)) ||
         (r = grow(ctx, err, fa, fa.clip_rec, clip_recs * sizeof(TriRec))) ||
         (r = grow(ctx, err, fa, fa.clip_verts, clip_recs * 3 * sizeof(Vertex56))) ||
@@ -163,10 +169,12 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs
         (r = grow(ctx, err, fa, fa.scan_sums, scan_blocks * 8)) ||
         (r = grow(ctx, err, fa, fa.vkeys, fs.pixels * 8)) ||
         (r = grow(ctx, err, fa, fa.big_rec, big_recs * sizeof(TriRec))) ||
-        (r = grow(ctx, err, fa, fa.big_start, big_recs * 4)))
+        (r = grow(ctx, err, fa, fa.big_start, big_recs * 4)) ||
+        (r = grow(ctx, err, fa, fa.vcol, fs.color_slots * 4)))
         return r;
     a.counters = (uint32_t*)fa.counters.p;
     a.rec = (TriRec*)fa.rec.p;
+    a.clip_base = (uint32_t*)fa.rec.p;
     a.clip_faces = (uint32_t*)fa.clip_faces.p;
     a.clip_rec = (TriRec*)fa.clip_rec.p;
     a.clip_verts = (Vertex56*)fa.clip_verts.p;
@@ -180,6 +188,7 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs
     a.vkeys = (unsigned long long*)fa.vkeys.p;
     a.big_rec = (TriRec*)fa.big_rec.p;
     a.big_start = (uint32_t*)fa.big_start.p;
+    a.vcol = (uint32_t*)fa.vcol.p;
     a.cap_clip_faces = (uint32_t)std::min<uint64_t>(fa.clip_faces.bytes / 4, 0xFFFFFFFFull);
     a.cap_clip_recs = (uint32_t)std::min<uint64_t>(
         {fa.clip_rec.bytes / sizeof(TriRec), fa.clip_verts.bytes / (3 * sizeof(Vertex56)), 0x7FFFFFFFull});
@@ -275,15 +284,16 @@ int arena_create(axr_ctx* ctx, ErrSink& err, FrameArena& fa) {
     return r ? r : arena_reset(err, fa);
 }
 
-static void arena_buffers(FrameArena& fa, DevBuf* (&out)[15]) {
-    DevBuf* bufs[15] = {&fa.counters, &fa.rec,       &fa.clip_faces, &fa.clip_rec, &fa.clip_verts,
-                        &fa.tile_count, &fa.tile_off, &fa.pairs,     &fa.wave_pairs, &fa.bins,
-                        &fa.active,   &fa.scan_sums,  &fa.vkeys,     &fa.big_rec,  &fa.big_start};
-    for (int i = 0; i < 15; ++i) out[i] = bufs[i];
+constexpr int kArenaBufs = 16;
+static void arena_buffers(FrameArena& fa, DevBuf* (&out)[kArenaBufs]) {
+    DevBuf* bufs[kArenaBufs] = {&fa.counters,  &fa.rec,      &fa.clip_faces, &fa.clip_rec,   &fa.clip_verts, &fa.tile_count,
+                                &fa.tile_off,  &fa.pairs,    &fa.wave_pairs, &fa.bins,       &fa.active,     &fa.scan_sums,
+                                &fa.vkeys,     &fa.big_rec,  &fa.big_start,  &fa.vcol};
+    for (int i = 0; i < kArenaBufs; ++i) out[i] = bufs[i];
 }
 
 uint64_t arena_bytes(const FrameArena& fa) {
-    DevBuf* bufs[15];
+    DevBuf* bufs[kArenaBufs];
     arena_buffers(const_cast<FrameArena&>(fa), bufs);
     uint64_t b = 0;
     for (DevBuf* d : bufs) b += d->bytes;
@@ -292,7 +302,7 @@ uint64_t arena_bytes(const FrameArena& fa) {
 
 void arena_destroy(axr_ctx* ctx, FrameArena& fa) {
     if (fa.stream) (void)hipStreamSynchronize(fa.stream);
-    DevBuf* bufs[15];
+    DevBuf* bufs[kArenaBufs];
     arena_buffers(fa, bufs);
     for (DevBuf* b : bufs) free_buf(ctx, *b);
     if (fa.host_counters) (void)hipHostFree(fa.host_counters);
@@ -315,11 +325,10 @@ static int arena_prep_if_dirty(ErrSink& err, FrameArena& fa) {
     return AXR_OK;
 }
 
-int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev, const FbWait& wait) {
+int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, StageEvents* ev, const FbWait& wait) {
     int r = arena_prep_if_dirty(err, fa);
     if (r) return r;
-    if (ev) (void)hipEventRecord(ev[0], fa.stream);
-    hipError_t e = launch_frame(ka, fa.a, fa.stream, ev ? ev + 1 : nullptr, wait);
+    hipError_t e = launch_frame(ka, fa.a, fa.stream, ev, wait);
     return e == hipSuccess ? AXR_OK : hip_fail(err, e, "kernel launch");
 }
 
@@ -454,27 +463,30 @@ namespace {
 void harvest_events(axr_ctx* ctx) {
     while (ctx->ev_pending > 0) {
         int slot = (ctx->ev_head - ctx->ev_pending + axr_ctx::kEventRing) % axr_ctx::kEventRing;
-        hipEvent_t* e = ctx->ev[slot];
-        (void)hipEventSynchronize(e[5]);
+        StageEvents& e = ctx->ev[slot];
+        --ctx->ev_pending;
+        if (!e.used) continue;   // the frame failed before its first kernel
+        const int first = __builtin_ctz(e.used), last = 31 - __builtin_clz(e.used);
+        if (hipEventSynchronize(e.stop[last]) != hipSuccess) continue;
         float total = 0;
-        (void)hipEventElapsedTime(&total, e[0], e[5]);
+        (void)hipEventElapsedTime(&total, e.start[first], e.stop[last]);
         ctx->stats.frames += 1;
         ctx->stats.frame_ms_total += total;
         for (int k = 0; k < 5; ++k) {
+            if (!((e.used >> k) & 1u)) continue;
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, e[k], e[k + 1]);
+            (void)hipEventElapsedTime(&ms, e.start[k], e.stop[k]);
             ctx->stats.stage_ms_total[k] += ms;
         }
-        --ctx->ev_pending;
     }
 }
 
-// Events for one frame: [0] before the frame, [1..5] after setup, clip,
-// scan, scatter and raster.
-hipEvent_t* next_events(axr_ctx* ctx) {
+// Kernel timing events for one frame (null when profiling is off).
+StageEvents* next_events(axr_ctx* ctx) {
     if (!ctx->profiling) return nullptr;
     if (ctx->ev_pending == axr_ctx::kEventRing) harvest_events(ctx);
-    hipEvent_t* ev = ctx->ev[ctx->ev_head];
+    StageEvents* ev = &ctx->ev[ctx->ev_head];
+    ev->used = 0;
     ctx->ev_head = (ctx->ev_head + 1) % axr_ctx::kEventRing;
     ++ctx->ev_pending;
     return ev;
@@ -487,6 +499,12 @@ int ctx_launch(axr_ctx* ctx, const KArgs& ka) { return arena_launch(ctx->err, ct
 extern "C" {
 
 const char* axr_version(void) { return "axiomr-mi355x 0.1 (gfx950)"; }
+
+int axr_check_view(uint64_t submitted_faces, int width, int height) {
+    if (submitted_faces >= MAX_VIEW_FACES) return AXR_ERR_INVALID;
+    if (width <= 0 || height <= 0 || width > MAX_FB_EDGE || height > MAX_FB_EDGE) return AXR_ERR_INVALID;
+    return AXR_OK;
+}
 
 int axr_device_count(int* count) {
     if (!count) return AXR_ERR_INVALID;
@@ -522,13 +540,21 @@ int axr_ctx_create(int device, uint64_t hbm_budget_bytes, axr_ctx** out) {
 int axr_ctx_destroy(axr_ctx* ctx) {
     if (!ctx) return AXR_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
+    // every stream first: a stream may still wait on another arena's signal,
+    // which returns to the shared free list when its arena is destroyed
+    if (ctx->main.stream) (void)ctx_sync_all(ctx);
     for (auto& a : ctx->alt) arena_destroy(ctx, a);
     arena_destroy(ctx, ctx->main);
     for (auto& e : ctx->join_ev)
         if (e) (void)hipEventDestroy(e);
-    for (auto& ring : ctx->ev)
-        for (auto& e : ring)
-            if (e) (void)hipEventDestroy(e);
+    if (ctx->ev) {
+        for (int i = 0; i < axr_ctx::kEventRing; ++i)
+            for (int k = 0; k < 5; ++k) {
+                if (ctx->ev[i].start[k]) (void)hipEventDestroy(ctx->ev[i].start[k]);
+                if (ctx->ev[i].stop[k]) (void)hipEventDestroy(ctx->ev[i].stop[k]);
+            }
+        delete[] ctx->ev;
+    }
     delete ctx;
     return AXR_OK;
 }
@@ -561,10 +587,14 @@ int axr_ctx_get_config(const axr_ctx* ctx, axr_config* cfg) {
 int axr_ctx_set_profiling(axr_ctx* ctx, int enable) {
     if (!ctx) return AXR_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
-    if (enable && !ctx->ev[0][0]) {
-        for (auto& ring : ctx->ev)
-            for (auto& e : ring)
-                if (hipEventCreate(&e) != hipSuccess) return fail(ctx->err, AXR_ERR_HIP, "hipEventCreate");
+    if (enable && !ctx->ev) {
+        ctx->ev = new (std::nothrow) StageEvents[axr_ctx::kEventRing];
+        if (!ctx->ev) return fail(ctx->err, AXR_ERR_OOM, "profiling events");
+        for (int i = 0; i < axr_ctx::kEventRing; ++i)
+            for (int k = 0; k < 5; ++k)
+                if (hipEventCreate(&ctx->ev[i].start[k]) != hipSuccess ||
+                    hipEventCreate(&ctx->ev[i].stop[k]) != hipSuccess)
+                    return fail(ctx->err, AXR_ERR_HIP, "hipEventCreate");
     }
     if (!enable) harvest_events(ctx);
     ctx->profiling = enable != 0;
@@ -671,7 +701,9 @@ int axr_mesh_create(axr_ctx* ctx, const float* verts56, uint64_t nverts, const u
         dg[g] = DevGroup{s.first_face, s.face_count, (uint32_t)rank, s.material};
         rank += s.face_count;
     }
-    if (rank > 0x1FFFFFFFull) return fail(err, AXR_ERR_INVALID, "mesh: too many faces in submission order");
+    if (rank >= MAX_VIEW_FACES)
+        return fail(err, AXR_ERR_INVALID, "mesh: %llu faces in submission order (the limit is 2^29 - 1)",
+                    (unsigned long long)rank);
     std::vector<DevMaterial> dm(nmaterials);
     for (uint32_t i = 0; i < nmaterials; ++i) {
         const axr_material& s = materials[i];
@@ -714,6 +746,9 @@ int axr_mesh_create(axr_ctx* ctx, const float* verts56, uint64_t nverts, const u
 
 int axr_mesh_destroy(axr_mesh* m) {
     if (!m) return AXR_ERR_INVALID;
+    if (m->batch_refs)
+        return fail(m->ctx->err, AXR_ERR_INVALID, "mesh is drawn by %u live batch view(s): destroy the batch first",
+                    m->batch_refs);
     (void)hipSetDevice(m->ctx->device);
     (void)ctx_sync_all(m->ctx);
     dev_free(m->ctx, m->pos, m->nverts * 12);
@@ -729,7 +764,7 @@ int axr_mesh_destroy(axr_mesh* m) {
 int axr_fb_create(axr_ctx* ctx, int w, int h, int with_ids, axr_fb** out) {
     if (!ctx || !out) return AXR_ERR_INVALID;
     *out = nullptr;
-    if (w <= 0 || h <= 0 || w > 32768 || h > 32768)
+    if (w <= 0 || h <= 0 || w > MAX_FB_EDGE || h > MAX_FB_EDGE)
         return fail(ctx->err, AXR_ERR_INVALID, "framebuffer: bad size %dx%d", w, h);
     (void)hipSetDevice(ctx->device);
     axr_fb* fb = new (std::nothrow) axr_fb();
@@ -758,6 +793,8 @@ int axr_fb_create(axr_ctx* ctx, int w, int h, int with_ids, axr_fb** out) {
 
 int axr_fb_destroy(axr_fb* fb) {
     if (!fb) return AXR_ERR_INVALID;
+    if (fb->batch_refs)
+        return fail(fb->ctx->err, AXR_ERR_INVALID, "framebuffer is a view of a live batch: destroy the batch first");
     (void)hipSetDevice(fb->ctx->device);
     (void)ctx_sync_all(fb->ctx);
     const size_t n = (size_t)fb->w * fb->h;
@@ -837,7 +874,7 @@ int axr_fb_write_png(axr_fb* fb, const char* path) {
 // on the stream of the latest framebuffer work (among it the previous
 // draw's framebuffer pass), so it overlaps earlier frames and still writes
 // framebuffers in call order.
-static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const FrameSize& fs) {
+static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const FrameSize& fs, StageEvents* ev) {
     const int x = ctx->next_arena;
     FrameArena& fa = arena_no(ctx, x);
     int r;
@@ -856,7 +893,7 @@ static int draw_pipelined(axr_ctx* ctx, axr_fb* fb, const KArgs& ka, const Frame
         }
     }
     if (x) ctx->alt_unjoined[x - 1] = true;   // before the launch: a failed one may have queued work
-    if ((r = arena_launch(ctx->err, fa, ka, nullptr, wait))) return r;
+    if ((r = arena_launch(ctx->err, fa, ka, ev, wait))) return r;
     fb->clear_pending = false;   // written by this launch, even by a frame that overflows
     // the end of this draw's framebuffer pass, for the next draw to wait on
     // (with last_value 0 the next draw waits with an event on this stream)
@@ -880,6 +917,9 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     if (fb->ctx != ctx || mesh->ctx != ctx)
         return fail(ctx->err, AXR_ERR_INVALID, "axr_draw: handle from another context");
     if (!shader_ok(p)) return fail(ctx->err, AXR_ERR_INVALID, "unknown shader %d", p->shader);
+    if (axr_check_view(mesh->nfaces_sub, fb->w, fb->h))
+        return fail(ctx->err, AXR_ERR_INVALID, "axr_draw: view too large (%u faces, %dx%d)", mesh->nfaces_sub, fb->w,
+                    fb->h);
     (void)hipSetDevice(ctx->device);
     const int tile = ctx->cfg.tile_size;
     KArgs ka{};
@@ -893,7 +933,6 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     ka.mode = ctx->cfg.raster_mode;
     ka.total_pixels = (uint32_t)((uint64_t)fb->w * (uint64_t)fb->h);
     ka.shader = p->shader;
-    const FrameSize fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
     if (fb->clear_pending && ka.mode == RASTER_VISIBILITY) {
         // the resolve pass writes the clear (k_resolve, ka.clear)
         ka.clear = 1;
@@ -901,12 +940,14 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
         ka.clear_depth = fb->clear_depth;
     }
     const bool async = (p->flags & AXR_DRAW_ASYNC) != 0;
-    // an asynchronous draw that absorbs a clear into a framebuffer without
-    // ids shades apart from its framebuffer writes (also when profiled, so
-    // the stage times are those of the same kernels)
-    ka.two_pass = async && ka.clear && !fb->ids;
-    if (async && ka.mode == RASTER_VISIBILITY && !ctx->profiling && overlap_enabled())
-        return draw_pipelined(ctx, fb, ka, fs);
+    // an asynchronous draw that absorbs a clear shades apart from its
+    // framebuffer writes (also when profiled, so the stage times are those
+    // of the same kernels); with an id plane the winners' colours wait in
+    // the arena's colour slots
+    ka.two_pass = async && ka.clear;
+    FrameSize fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
+    if (ka.two_pass && fb->ids) fs.color_slots = fs.pixels;
+    if (async && ka.mode == RASTER_VISIBILITY && overlap_enabled()) return draw_pipelined(ctx, fb, ka, fs, next_events(ctx));
     int r;
     if (fb->clear_pending && !ka.clear) {
         // binned mode: the clear is written now, so the winners are tested
@@ -947,10 +988,25 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     std::vector<uint32_t> fbase(nviews + 1), tbase(nviews + 1), pbase(nviews + 1);
     uint64_t faces = 0, tiles = 0, pixels = 0;
     const int tile = ctx->cfg.tile_size;
+    // every view writes its framebuffer in the same launch, with no order
+    // between views: two views of one framebuffer would race on its pixels
+    // (the reference's render loop draws meshes into one framebuffer one
+    // drawMesh after the other, renderer.cpp:100-110; here that is one view
+    // per framebuffer, or one batch per mesh)
+    std::vector<const axr_fb*> seen(fbs, fbs + nviews);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) {
+        delete b;
+        return fail(ctx->err, AXR_ERR_INVALID, "batch: a framebuffer appears in more than one view");
+    }
     for (uint32_t i = 0; i < nviews; ++i) {
         if (!fbs[i] || !meshes[i] || fbs[i]->ctx != ctx || meshes[i]->ctx != ctx || !shader_ok(&params[i])) {
             delete b;
             return fail(ctx->err, AXR_ERR_INVALID, "batch: view %u has a bad handle or shader", i);
+        }
+        if (axr_check_view(meshes[i]->nfaces_sub, fbs[i]->w, fbs[i]->h)) {
+            delete b;
+            return fail(ctx->err, AXR_ERR_INVALID, "batch: view %u too large (%u faces)", i, meshes[i]->nfaces_sub);
         }
         const axr_fb* fb = fbs[i];
         ViewDesc v = make_view(meshes[i], fb->color, fb->depth, fb->ids, fb->w, fb->h, &params[i], tile);
@@ -963,7 +1019,7 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
         faces += v.nfaces;
         tiles += (uint64_t)v.tiles_x * v.tiles_y;
         pixels += (uint64_t)v.W * v.H;
-        if (faces > 0x7FFFFFFFull || tiles > 0x7FFFFFFFull || pixels > 0xFFFFFFFFull) {
+        if (faces > MAX_BATCH_FACES || tiles > 0x7FFFFFFFull || pixels > 0xFFFFFFFFull) {
             delete b;
             return fail(ctx->err, AXR_ERR_INVALID, "batch too large");
         }
@@ -1011,6 +1067,14 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     ka.one = b->views[0];
     b->fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
     if ((r = arena_ensure(ctx, err, ctx->main, b->fs))) { axr_batch_destroy(b); return r; }
+    // the batch's tables and captured graph hold the views' device pointers:
+    // their framebuffers and meshes cannot be destroyed before the batch
+    for (uint32_t i = 0; i < nviews; ++i) {
+        ++fbs[i]->batch_refs;
+        ++const_cast<axr_mesh*>(meshes[i])->batch_refs;
+    }
+    b->meshes.assign(meshes, meshes + nviews);
+    b->holds_refs = true;
     *out = b;
     return AXR_OK;
 }
@@ -1098,6 +1162,10 @@ int axr_batch_destroy(axr_batch* b) {
     dev_free(ctx, b->d_tile_bases, sz_b);
     dev_free(ctx, b->d_pix_bases, sz_b);
     dev_free(ctx, b->d_tile_views, (size_t)b->ka.total_tiles * 4);
+    if (b->holds_refs) {
+        for (axr_fb* fb : b->fbs) --fb->batch_refs;
+        for (const axr_mesh* m : b->meshes) --const_cast<axr_mesh*>(m)->batch_refs;
+    }
     delete b;
     return AXR_OK;
 }

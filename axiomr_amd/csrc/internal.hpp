@@ -21,10 +21,21 @@ struct FbWait {
     uint64_t value = 0;
 };
 
+// Kernel timing of one frame: stage k's first kernel carries start[k] and
+// its last kernel stop[k], attached to the dispatches themselves
+// (hipExtLaunchKernel), so stop - start is the stage's kernel time as the
+// command processor sees it (the figure a kernel trace reports), without
+// the launch gap before it.  `used` has bit k set for every stage launched.
+struct StageEvents {
+    hipEvent_t start[5] = {};
+    hipEvent_t stop[5] = {};
+    uint32_t used = 0;
+};
+
 // wait (optional): what the stream waits for before the pass that writes
 // framebuffers (k_resolve / k_finish / k_raster); the passes before it only
-// touch the frame arena and the meshes.
-hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& wait = {});
+// touch the frame arena and the meshes.  ev (optional): kernel timing.
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& wait = {});
 hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra, float d, hipStream_t st);
 hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st);
 hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
@@ -52,7 +63,7 @@ struct ArenaNeeds {
 struct FrameArena {
     hipStream_t stream = nullptr;
     DevBuf counters, rec, clip_faces, clip_rec, clip_verts, tile_count, tile_off, pairs, wave_pairs, bins, active,
-        scan_sums, vkeys, big_rec, big_start;
+        scan_sums, vkeys, big_rec, big_start, vcol;
     Arena a{};
     uint64_t gen = 0;          // bumped whenever a buffer moves: captured graphs are stale
     bool dirty = true;         // bin counts / counters need a k_prep before the next frame
@@ -88,8 +99,10 @@ struct axr_ctx {
     uint64_t last_value = 0;            // nonzero: that work was a draw whose stream then wrote this
                                         // value to its arena's signal (0: wait with an event)
     bool profiling = false;
-    static constexpr int kEventRing = 64;
-    hipEvent_t ev[kEventRing][6] = {};
+    // timed frames not yet read back; the ring holds the frames of a whole
+    // timed run, so timing does not stall the host until stats are read
+    static constexpr int kEventRing = 1024;
+    axr::StageEvents* ev = nullptr;     // kEventRing entries, created on first use
     int ev_head = 0, ev_pending = 0;
     axr_kernel_stats stats{};
 };
@@ -111,6 +124,7 @@ struct axr_mesh {
     uint32_t ngroups = 0, nmaterials = 0;
     uint64_t nverts = 0, nfaces = 0;
     uint32_t nfaces_sub = 0;   // faces in submission order
+    uint32_t batch_refs = 0;   // views of live batches that draw this mesh
     axr::DevGroup g0{};        // first group (host copy)
     size_t bytes = 0;
 };
@@ -122,6 +136,7 @@ struct axr_fb {
     float* depth = nullptr;
     uint32_t* ids = nullptr;
     size_t bytes = 0;
+    uint32_t batch_refs = 0;   // live batches with this framebuffer as a view
     // A clear not yet written: the next visibility-mode draw writes it as
     // part of its resolve; anything else that reads or draws into the planes
     // writes it first (axr::fb_settle).
@@ -144,6 +159,7 @@ struct FrameSize {
     uint64_t faces = 0;      // face slots over all views
     uint64_t tiles = 0;      // bins over all views
     uint64_t pixels = 0;     // visibility-buffer pixels (0 in binned mode)
+    uint64_t color_slots = 0; // pixels of a two-pass resolve into a framebuffer with ids
     uint32_t wave_cap = 128; // pairs per setup-wave region (binned mode)
 };
 // High-water marks the kernels reported after an overflow.
@@ -155,8 +171,8 @@ uint64_t arena_bytes(const FrameArena& fa);
 int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs, const ArenaNeeds& need = {});
 int arena_read_counters(ErrSink& err, FrameArena& fa);
 // k_prep if the arena is fresh or dirty, then the frame's kernels; ev (may
-// be null) receives the 5 stage events.
-int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, hipEvent_t* ev, const FbWait& wait = {});
+// be null) times them.
+int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, StageEvents* ev, const FbWait& wait = {});
 // After arena_read_counters: AXR_OK, or grow the arena to the reported
 // high-water marks, reset it and return AXR_ERR_OVERFLOW.
 int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs);

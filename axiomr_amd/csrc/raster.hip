@@ -22,6 +22,7 @@
 #include "axr_types.hpp"
 #include "shade.hpp"
 #include "internal.hpp"
+#include <hip/hip_ext.h>
 
 #pragma clang fp contract(off)
 
@@ -722,11 +723,15 @@ __global__ __launch_bounds__(64) void k_clip(KArgs ka, Arena ar) {
                 emit_pairs_lane<T>(ar, V, rc, REF_CLIP | slot);
             }
         }
-        TriRec m;
-        m.a = make_float4(0, 0, 0, 0);
-        m.b = make_float4(0, 0, 0, 0);
-        m.c = make_uint4(0u, s * 8u, base, REC_CLIPPED | (G.material & REC_MATERIAL_MASK));
-        ar.rec[g] = m;
+        if constexpr (VIS) {
+            ar.clip_base[g] = base;
+        } else {
+            TriRec m;
+            m.a = make_float4(0, 0, 0, 0);
+            m.b = make_float4(0, 0, 0, 0);
+            m.c = make_uint4(0u, s * 8u, base, REC_CLIPPED | (G.material & REC_MATERIAL_MASK));
+            ar.rec[g] = m;
+        }
     }
 }
 
@@ -1203,7 +1208,7 @@ __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& 
     uint32_t slot = 0;
     if (classify(c0, c1, c2) == CLS_CLIP) {
         clipped = true;
-        slot = ar.rec[V.face_base + s].c.z + j;   // redirect written by k_clip
+        slot = ar.clip_base[V.face_base + s] + j;   // written by k_clip
         r = ar.clip_rec[slot];
     } else {
         r = make_rec(c0, c1, c2, V.W, V.H);
@@ -1375,11 +1380,11 @@ __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
 // k_shade shades every winner into its own key slot (depth bits << 32 |
 // packed colour; a winner behind the clear empties the slot); k_finish, the
 // only pass that touches the framebuffer, writes every pixel from its slot
-// or the clear and leaves the slots and bin flags empty.  A framebuffer
-// with an id plane takes the one-pass k_resolve (the slot has no room for
-// the winner's id).
+// or the clear and leaves the slots and bin flags empty.  For a framebuffer
+// with an id plane (IDS) the slot keeps the winner's key in its low word
+// and the colour goes to the arena's colour slots (vcol).
 
-template <int T, int SK>
+template <int T, int SK, bool IDS>
 __global__ __launch_bounds__(256) void k_shade(KArgs ka, Arena ar) {
     using S = typename ShaderOf<SK>::type;
     if (ar.counters[CNT_ERROR]) return;
@@ -1411,14 +1416,15 @@ __global__ __launch_bounds__(256) void k_shade(KArgs ka, Arena ar) {
         float z;
         uint32_t bgra;
         const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra);
-        vk[pix] = drawn ? ((unsigned long long)__float_as_uint(z) << 32) | bgra : EMPTY_KEY;
+        vk[pix] = drawn ? ((unsigned long long)__float_as_uint(z) << 32) | (IDS ? (uint32_t)k : bgra) : EMPTY_KEY;
+        if (IDS && drawn) ar.vcol[V.pix_base + pix] = bgra;
     }
 }
 
 // One wave per strip of 64 pixels by one bin row (64 / T bins side by
 // side), one lane per column: each row of the strip is one 256-byte store
 // per plane.  A slot is read only in bins that had keys.
-template <int T>
+template <int T, bool IDS>
 __global__ __launch_bounds__(256) void k_finish(KArgs ka, Arena ar) {
     static_assert(64 % T == 0, "a strip holds whole bins");
     const bool failed = ar.counters[CNT_ERROR] != 0u;
@@ -1452,8 +1458,9 @@ __global__ __launch_bounds__(256) void k_finish(KArgs ka, Arena ar) {
                 const size_t pix = pix0 + (size_t)(r0 + r) * V.W;
                 if (sl[r] != EMPTY_KEY) {
                     vk[pix] = EMPTY_KEY;
-                    V.color[pix] = (uint32_t)sl[r];
+                    V.color[pix] = IDS ? ar.vcol[V.pix_base + pix] : (uint32_t)sl[r];
                     V.depth[pix] = __uint_as_float((uint32_t)(sl[r] >> 32));
+                    if (IDS) V.ids[pix] = (uint32_t)sl[r];
                 } else {
                     clear_pixel(V, pix, cs);
                 }
@@ -1469,7 +1476,7 @@ __global__ __launch_bounds__(256) void k_finish(KArgs ka, Arena ar) {
 // 64 pixels by one bin row, each lane four adjacent pixels of every fourth
 // row, 16-byte loads and stores (each store instruction covers four rows
 // of 256 contiguous bytes).
-template <int T>
+template <int T, bool IDS>
 __global__ __launch_bounds__(256) void k_finish4(KArgs ka, Arena ar) {
     static_assert(64 % T == 0 && T % 4 == 0, "a strip holds whole bins, a lane's pixels one bin");
     constexpr int RL = T / 4;   // rows per lane
@@ -1512,17 +1519,28 @@ __global__ __launch_bounds__(256) void k_finish4(KArgs ka, Arena ar) {
             const unsigned long long s0 = a[k].x, s1 = a[k].y, s2 = b[k].x, s3 = b[k].y;
             uint4 c = cv;
             float4 d = dv;
+            uint4 id = make_uint4(~0u, ~0u, ~0u, ~0u);
             if ((s0 & s1 & s2 & s3) != EMPTY_KEY) {
                 if (s0 != EMPTY_KEY) { c.x = (uint32_t)s0; d.x = __uint_as_float((uint32_t)(s0 >> 32)); }
                 if (s1 != EMPTY_KEY) { c.y = (uint32_t)s1; d.y = __uint_as_float((uint32_t)(s1 >> 32)); }
                 if (s2 != EMPTY_KEY) { c.z = (uint32_t)s2; d.z = __uint_as_float((uint32_t)(s2 >> 32)); }
                 if (s3 != EMPTY_KEY) { c.w = (uint32_t)s3; d.w = __uint_as_float((uint32_t)(s3 >> 32)); }
+                if constexpr (IDS) {
+                    // the slots hold the winners' keys; their colours are in vcol
+                    id = c;
+                    const uint4 vc = *reinterpret_cast<const uint4*>(ar.vcol + V.pix_base + pix);
+                    c = make_uint4(s0 != EMPTY_KEY ? vc.x : cv.x, s1 != EMPTY_KEY ? vc.y : cv.y,
+                                   s2 != EMPTY_KEY ? vc.z : cv.z, s3 != EMPTY_KEY ? vc.w : cv.w);
+                    id = make_uint4(s0 != EMPTY_KEY ? id.x : ~0u, s1 != EMPTY_KEY ? id.y : ~0u,
+                                    s2 != EMPTY_KEY ? id.z : ~0u, s3 != EMPTY_KEY ? id.w : ~0u);
+                }
                 ulonglong2* q = reinterpret_cast<ulonglong2*>(vk + pix);
                 q[0] = ev2;
                 q[1] = ev2;
             }
             *reinterpret_cast<uint4*>(V.color + pix) = c;
             *reinterpret_cast<float4*>(V.depth + pix) = d;
+            if constexpr (IDS) *reinterpret_cast<uint4*>(V.ids + pix) = id;
         }
         // every lane of a bin read its flag before the loop's loads returned
         if (keys && rg == 0u && (uint32_t)px % (uint32_t)T == 0u) ar.tile_count[t] = 0u;
@@ -1571,119 +1589,127 @@ __global__ void k_clear_views(const ViewDesc* views, uint32_t view0, uint32_t bg
 // ---------------------------------------------------------------------------
 // host-side launch wrappers
 
-struct LaunchShape {
-    uint32_t setup_blocks, clip_blocks, scatter_blocks, scan_blocks;
-};
-
-// ev (optional): 5 events recorded after setup, clip, scan, scatter and
-// raster, on the stream the kernels run on.  Expects a clean arena (bin
-// counts and per-frame counters zero) and leaves it clean.
+// Expects a clean arena (bin counts and per-frame counters zero) and leaves
+// it clean.
 constexpr int RESOLVE_MAX_BLOCKS = 16384;
 
-// Visibility mode: k_vis, k_clip, k_big, k_resolve (or k_shade, k_finish
-// for a two-pass draw).  Events: after k_vis, k_clip, k_big, (nothing or
-// k_shade), k_resolve (or k_finish).
+// One kernel of stage `stage`.  With kernel timing (ev), the first kernel
+// of a stage carries the stage's start event and its last kernel the stop
+// event, attached to the dispatches (hipExtLaunchKernel).
+template <class... KA, class... A>
+static void stage_launch(void (*kern)(KA...), dim3 grid, dim3 block, hipStream_t st, StageEvents* ev, int stage,
+                         bool first, bool last, A... args) {
+    if (ev) {
+        hipExtLaunchKernelGGL(kern, grid, block, 0, st, first ? ev->start[stage] : nullptr,
+                              last ? ev->stop[stage] : nullptr, 0, args...);
+        ev->used |= 1u << stage;
+    } else {
+        hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
+    }
+}
+template <class... KA, class... A>
+static void stage_launch1(void (*kern)(KA...), dim3 grid, dim3 block, hipStream_t st, StageEvents* ev, int stage,
+                          A... args) {
+    stage_launch(kern, grid, block, st, ev, stage, true, true, args...);
+}
+
 static hipError_t fb_wait(hipStream_t st, const FbWait& w) {
     if (w.sig) return hipStreamWaitValue64(st, w.sig, w.value, hipStreamWaitValueGte);
     if (w.ev) return hipStreamWaitEvent(st, w.ev, 0);
     return hipSuccess;
 }
 
-template <int T, bool BATCH>
-static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& wait) {
-    const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
-    if (vis_blocks) k_vis<T, BATCH><<<vis_blocks, 256, 0, st>>>(ka, ar);
-    if (ev) (void)hipEventRecord(ev[0], st);
-    k_clip<T, BATCH, true><<<256, 64, 0, st>>>(ka, ar);
-    if (ev) (void)hipEventRecord(ev[1], st);
-    k_big<T, BATCH><<<1024, 256, 0, st>>>(ka, ar);
-    if (ev) (void)hipEventRecord(ev[2], st);
-    if (!BATCH && ka.two_pass && ka.clear && !ka.one.ids && ka.shader >= 0 && ka.shader < SHADER_COUNT) {
-        // shade at once, write the framebuffer after the earlier draws
-        const uint64_t units = (uint64_t)ka.total_tiles * (uint64_t)((T * T + 63) / 64);
-        const int sb = (int)std::min<uint64_t>(std::max<uint64_t>(1, (units + 3) / 4), RESOLVE_MAX_BLOCKS);
-        switch (ka.shader) {
-        case SHADER_FLAT: k_shade<T, SHADER_FLAT><<<sb, 256, 0, st>>>(ka, ar); break;
-        case SHADER_PHONG: k_shade<T, SHADER_PHONG><<<sb, 256, 0, st>>>(ka, ar); break;
-        case SHADER_PBR: k_shade<T, SHADER_PBR><<<sb, 256, 0, st>>>(ka, ar); break;
-        case SHADER_GOURAUD: k_shade<T, SHADER_GOURAUD><<<sb, 256, 0, st>>>(ka, ar); break;
-        case SHADER_PHONG_POINT: k_shade<T, SHADER_PHONG_POINT><<<sb, 256, 0, st>>>(ka, ar); break;
-        default: k_shade<T, SHADER_FLAT_COLOR><<<sb, 256, 0, st>>>(ka, ar); break;
-        }
-        if (ev) (void)hipEventRecord(ev[3], st);
-        hipError_t e = fb_wait(st, wait);
-        if (e != hipSuccess) return e;
-        if (ka.one.W % 4 == 0) {
-            const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
-            k_finish4<T><<<imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
-        } else {
-            const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
-            k_finish<T><<<imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS)), 256, 0, st>>>(ka, ar);
-        }
-        if (ev) (void)hipEventRecord(ev[4], st);
-        return hipGetLastError();
+// The shader-specialised kernel of ka.shader (-1: the switching one).
+template <template <int> class K>
+static auto by_shader(int shader) -> decltype(K<-1>::fn) {
+    switch (shader) {
+    case SHADER_FLAT: return K<SHADER_FLAT>::fn;
+    case SHADER_PHONG: return K<SHADER_PHONG>::fn;
+    case SHADER_PBR: return K<SHADER_PBR>::fn;
+    case SHADER_GOURAUD: return K<SHADER_GOURAUD>::fn;
+    case SHADER_PHONG_POINT: return K<SHADER_PHONG_POINT>::fn;
+    case SHADER_FLAT_COLOR: return K<SHADER_FLAT_COLOR>::fn;
+    default: return K<-1>::fn;
     }
-    if (ev) (void)hipEventRecord(ev[3], st);
+}
+template <int T, bool IDS> struct ShadeK {
+    template <int SK> struct K { static constexpr auto fn = &k_shade<T, (SK < 0 ? SHADER_FLAT_COLOR : SK), IDS>; };
+};
+template <int T, bool BATCH> struct ResolveK {
+    template <int SK> struct K { static constexpr auto fn = &k_resolve<T, BATCH, SK>; };
+};
+template <int T, int NT, bool BATCH> struct RasterK {
+    template <int SK> struct K { static constexpr auto fn = &k_raster<T, NT, BATCH, SK>; };
+};
+
+// k_shade, then (after the wait) k_finish4 or k_finish: stages 3 and 4.
+template <int T, bool IDS>
+static hipError_t launch_two_pass(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev,
+                                  const FbWait& wait) {
+    const uint64_t units = (uint64_t)ka.total_tiles * (uint64_t)((T * T + 63) / 64);
+    const int sb = (int)std::min<uint64_t>(std::max<uint64_t>(1, (units + 3) / 4), RESOLVE_MAX_BLOCKS);
+    stage_launch1(by_shader<ShadeK<T, IDS>::template K>(ka.shader), dim3(sb), dim3(256), st, ev, 3, ka, ar);
+    hipError_t e = fb_wait(st, wait);
+    if (e != hipSuccess) return e;
+    const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
+    const int fblocks = imax(1, imin((int)((strips + 3) / 4), RESOLVE_MAX_BLOCKS));
+    if (ka.one.W % 4 == 0) stage_launch1(k_finish4<T, IDS>, dim3(fblocks), dim3(256), st, ev, 4, ka, ar);
+    else stage_launch1(k_finish<T, IDS>, dim3(fblocks), dim3(256), st, ev, 4, ka, ar);
+    return hipGetLastError();
+}
+
+// Visibility mode: stage 0 k_vis, 1 k_clip, 2 k_big, then either 4
+// k_resolve or (two-pass) 3 k_shade and 4 k_finish.
+template <int T, bool BATCH>
+static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& wait) {
+    const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
+    if (vis_blocks) stage_launch1(k_vis<T, BATCH>, dim3(vis_blocks), dim3(256), st, ev, 0, ka, ar);
+    stage_launch1(k_clip<T, BATCH, true>, dim3(256), dim3(64), st, ev, 1, ka, ar);
+    stage_launch1(k_big<T, BATCH>, dim3(1024), dim3(256), st, ev, 2, ka, ar);
+    if (!BATCH && ka.two_pass && ka.clear && ka.shader >= 0 && ka.shader < SHADER_COUNT) {
+        // shade at once, write the framebuffer after the earlier draws
+        return ka.one.ids ? launch_two_pass<T, true>(ka, ar, st, ev, wait)
+                          : launch_two_pass<T, false>(ka, ar, st, ev, wait);
+    }
     // the framebuffer pass waits for what other streams were asked to do first
     hipError_t e = fb_wait(st, wait);
     if (e != hipSuccess) return e;
     // T = 8: four bins per workgroup; larger bins: one bin per workgroup
     const uint32_t per_block = T == 8 ? 4u : 1u;
     const int rb = imax(1, imin((int)((ka.total_tiles + per_block - 1) / per_block), RESOLVE_MAX_BLOCKS));
-    switch (ka.shader) {
-    case SHADER_FLAT: k_resolve<T, BATCH, SHADER_FLAT><<<rb, 256, 0, st>>>(ka, ar); break;
-    case SHADER_PHONG: k_resolve<T, BATCH, SHADER_PHONG><<<rb, 256, 0, st>>>(ka, ar); break;
-    case SHADER_PBR: k_resolve<T, BATCH, SHADER_PBR><<<rb, 256, 0, st>>>(ka, ar); break;
-    case SHADER_GOURAUD: k_resolve<T, BATCH, SHADER_GOURAUD><<<rb, 256, 0, st>>>(ka, ar); break;
-    case SHADER_PHONG_POINT: k_resolve<T, BATCH, SHADER_PHONG_POINT><<<rb, 256, 0, st>>>(ka, ar); break;
-    case SHADER_FLAT_COLOR: k_resolve<T, BATCH, SHADER_FLAT_COLOR><<<rb, 256, 0, st>>>(ka, ar); break;
-    default: k_resolve<T, BATCH, -1><<<rb, 256, 0, st>>>(ka, ar); break;
-    }
-    if (ev) (void)hipEventRecord(ev[4], st);
+    stage_launch1(by_shader<ResolveK<T, BATCH>::template K>(ka.shader), dim3(rb), dim3(256), st, ev, 4, ka, ar);
     return hipGetLastError();
 }
 
+// Binned mode: stage 0 k_setup, 1 k_clip, 2 the scan, 3 k_scatter, 4 k_raster.
 template <int T, int NT, bool BATCH>
-static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev,
+static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev,
                                  const FbWait& wait) {
     if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, BATCH>(ka, ar, st, ev, wait);
     const uint32_t ntiles = ka.total_tiles;
     const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
-    if (setup_blocks) k_setup<T, BATCH><<<setup_blocks, 256, 0, st>>>(ka, ar);
-    if (ev) (void)hipEventRecord(ev[0], st);
-    k_clip<T, BATCH, false><<<256, 64, 0, st>>>(ka, ar);
-    if (ev) (void)hipEventRecord(ev[1], st);
+    if (setup_blocks) stage_launch1(k_setup<T, BATCH>, dim3(setup_blocks), dim3(256), st, ev, 0, ka, ar);
+    stage_launch1(k_clip<T, BATCH, false>, dim3(256), dim3(64), st, ev, 1, ka, ar);
     if (ntiles <= SMALL_SCAN_TILES) {
-        k_scan_single<<<1, SCAN_THREADS, 0, st>>>(ar, ntiles);
+        stage_launch1(k_scan_single, dim3(1), dim3(SCAN_THREADS), st, ev, 2, ar, ntiles);
     } else {
         const uint32_t scan_blocks = (ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
-        k_scan_reduce<<<scan_blocks, SCAN_THREADS, 0, st>>>(ar, ntiles);
-        k_scan_top<<<1, SCAN_THREADS, 0, st>>>(ar, scan_blocks);
-        k_scan_down<<<scan_blocks, SCAN_THREADS, 0, st>>>(ar, ntiles);
+        stage_launch(k_scan_reduce, dim3(scan_blocks), dim3(SCAN_THREADS), st, ev, 2, true, false, ar, ntiles);
+        stage_launch(k_scan_top, dim3(1), dim3(SCAN_THREADS), st, ev, 2, false, false, ar, scan_blocks);
+        stage_launch(k_scan_down, dim3(scan_blocks), dim3(SCAN_THREADS), st, ev, 2, false, true, ar, ntiles);
     }
-    if (ev) (void)hipEventRecord(ev[2], st);
     const uint32_t nwaves = setup_blocks * 4;
     const uint32_t sblocks = imax(64, imin(8192, (int)((nwaves + 3) / 4)));
-    k_scatter<<<sblocks, 256, 0, st>>>(ar, nwaves);
-    if (ev) (void)hipEventRecord(ev[3], st);
+    stage_launch1(k_scatter, dim3(sblocks), dim3(256), st, ev, 3, ar, nwaves);
     const uint32_t rblocks = ((imax(1, (int)ntiles) + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
     hipError_t e = fb_wait(st, wait);
     if (e != hipSuccess) return e;
-    switch (ka.shader) {
-    case SHADER_FLAT: k_raster<T, NT, BATCH, SHADER_FLAT><<<rblocks, NT, 0, st>>>(ka, ar); break;
-    case SHADER_PHONG: k_raster<T, NT, BATCH, SHADER_PHONG><<<rblocks, NT, 0, st>>>(ka, ar); break;
-    case SHADER_PBR: k_raster<T, NT, BATCH, SHADER_PBR><<<rblocks, NT, 0, st>>>(ka, ar); break;
-    case SHADER_GOURAUD: k_raster<T, NT, BATCH, SHADER_GOURAUD><<<rblocks, NT, 0, st>>>(ka, ar); break;
-    case SHADER_PHONG_POINT: k_raster<T, NT, BATCH, SHADER_PHONG_POINT><<<rblocks, NT, 0, st>>>(ka, ar); break;
-    case SHADER_FLAT_COLOR: k_raster<T, NT, BATCH, SHADER_FLAT_COLOR><<<rblocks, NT, 0, st>>>(ka, ar); break;
-    default: k_raster<T, NT, BATCH, -1><<<rblocks, NT, 0, st>>>(ka, ar); break;
-    }
-    if (ev) (void)hipEventRecord(ev[4], st);
+    stage_launch1(by_shader<RasterK<T, NT, BATCH>::template K>(ka.shader), dim3(rblocks), dim3(NT), st, ev, 4, ka, ar);
     return hipGetLastError();
 }
 
 template <int T, bool BATCH>
-static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& w) {
+static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& w) {
     switch (ka.raster_threads) {
     case 64: return launch_frame_t<T, 64, BATCH>(ka, ar, st, ev, w);
     case 128: return launch_frame_t<T, 128, BATCH>(ka, ar, st, ev, w);
@@ -1692,7 +1718,7 @@ static hipError_t launch_frame_nt(const KArgs& ka, const Arena& ar, hipStream_t 
 }
 
 template <bool BATCH>
-static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& w) {
+static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& w) {
     switch (ka.tile_size) {
     case 8: return launch_frame_nt<8, BATCH>(ka, ar, st, ev, w);
     case 32: return launch_frame_nt<32, BATCH>(ka, ar, st, ev, w);
@@ -1700,7 +1726,7 @@ static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t s
     }
 }
 
-hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, hipEvent_t* ev, const FbWait& wait) {
+hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& wait) {
     return ka.table ? launch_frame_b<true>(ka, ar, st, ev, wait) : launch_frame_b<false>(ka, ar, st, ev, wait);
 }
 

@@ -78,11 +78,13 @@ public:
         for (Lane& l : lanes_) {
             int r = arena_create(ctx_, err_, l.fa);
             if (r) return r;
-            // sized for a full batch up front so that serving never grows it
+            // sized for a full batch up front (max_batch_faces faces over
+            // max_batch_pixels pixels) so that serving does not grow it; only
+            // a single request above those caps, which runs alone, grows it
             FrameSize fs;
             fs.faces = cfg_.max_batch_faces;
-            fs.tiles = cfg_.max_batch_tiles;
-            fs.pixels = mode_ == RASTER_VISIBILITY ? cfg_.max_batch_tiles * (uint64_t)(tile_ * tile_) : 0;
+            fs.tiles = std::min<uint64_t>(cfg_.max_batch_tiles, lane_tiles());
+            fs.pixels = mode_ == RASTER_VISIBILITY ? cfg_.max_batch_pixels : 0;
             fs.wave_cap = wave_cap_;
             if ((r = arena_ensure(ctx_, err_, l.fa, fs))) return r;
             if ((r = dev_alloc(ctx_, err_, (void**)&l.d_views, views_bytes())) ||
@@ -134,7 +136,7 @@ public:
             tiles += (uint64_t)v.tiles_x * v.tiles_y;
             pixels += (uint64_t)q.w * q.h;
         }
-        if (faces > 0x7FFFFFFFull || tiles > cfg_.max_batch_tiles || pixels > 0xFFFFFFFFull)
+        if (faces > MAX_BATCH_FACES || tiles > cfg_.max_batch_tiles || pixels > 0xFFFFFFFFull)
             return fail(err_, AXR_ERR_INVALID, "batch too large (%llu faces, %llu bins)", (unsigned long long)faces,
                         (unsigned long long)tiles);
         L.h_bases[n] = (uint32_t)faces;
@@ -220,6 +222,12 @@ private:
         return e == hipSuccess ? AXR_OK : hip_fail(err_, e, "server batch enqueue");
     }
 
+    // Bins of a batch at the pixel cap: every pixel in bins of T x T, plus
+    // the partial bins along two edges of each view.
+    uint64_t lane_tiles() const {
+        const uint64_t t2 = (uint64_t)tile_ * tile_;
+        return cfg_.max_batch_pixels / t2 + (uint64_t)cfg_.max_batch_views * (2 * (MAX_FB_EDGE / tile_) + 1);
+    }
     size_t views_bytes() const { return (size_t)cfg_.max_batch_views * sizeof(ViewDesc); }
     size_t bases_bytes() const { return (size_t)3 * (cfg_.max_batch_views + 1) * 4; }
     size_t tiles_bytes() const { return (size_t)cfg_.max_batch_tiles * 4; }
@@ -251,16 +259,18 @@ int server_fail(axr_server* s, int code, const std::string& msg) {
 }
 
 // Validates one request; fills the admission sizes.
-int check_request(axr_server* s, const axr_serve_request* q, uint64_t* bytes, uint64_t* faces, uint64_t* tiles) {
+int check_request(axr_server* s, const axr_serve_request* q, uint64_t* bytes, uint64_t* faces, uint64_t* tiles,
+                  uint64_t* pixels) {
     if (!q || !q->mesh) return server_fail(s, AXR_ERR_INVALID, "request without a mesh");
     if (q->mesh->ctx != s->ctx) return server_fail(s, AXR_ERR_INVALID, "request mesh from another context");
-    if (q->width <= 0 || q->height <= 0 || q->width > 32768 || q->height > 32768)
-        return server_fail(s, AXR_ERR_INVALID, "request framebuffer size out of range");
+    if (axr_check_view(q->mesh->nfaces_sub, q->width, q->height))
+        return server_fail(s, AXR_ERR_INVALID, "request framebuffer size or face count out of range");
     if (!shader_ok(&q->params)) return server_fail(s, AXR_ERR_INVALID, "request names an unknown shader");
     const int t = s->dev->tile();
     *bytes = fb_bytes(q->width, q->height, s->cfg.with_ids != 0);
     *faces = q->mesh->nfaces_sub;
     *tiles = (uint64_t)((q->width + t - 1) / t) * ((q->height + t - 1) / t);
+    *pixels = (uint64_t)q->width * (uint64_t)q->height;
     if (*tiles > s->cfg.max_batch_tiles)
         return server_fail(s, AXR_ERR_INVALID, "request has more bins than max_batch_tiles");
     return AXR_OK;
@@ -279,9 +289,11 @@ int axr_server_create(axr_ctx* ctx, const axr_server_config* cfg_in, axr_server*
     if (cfg.max_batch_views == 0) cfg.max_batch_views = 64;
     if (cfg.max_batch_faces == 0) cfg.max_batch_faces = 8ull << 20;
     if (cfg.max_batch_tiles == 0) cfg.max_batch_tiles = 1ull << 20;
+    if (cfg.max_batch_pixels == 0) cfg.max_batch_pixels = 32ull << 20;
     if (cfg.lanes < 1 || cfg.lanes > 8)
         return fail(ctx->err, AXR_ERR_INVALID, "server lanes must be in [1, 8] (got %d)", cfg.lanes);
-    if (cfg.max_batch_views > 1024 || cfg.max_batch_faces > 0x40000000ull || cfg.max_batch_tiles > 0x10000000ull)
+    if (cfg.max_batch_views > 1024 || cfg.max_batch_faces > 0x40000000ull || cfg.max_batch_tiles > 0x10000000ull ||
+        cfg.max_batch_pixels > 0xFFFFFFFFull)
         return fail(ctx->err, AXR_ERR_INVALID, "server batch limits too large");
     (void)hipSetDevice(ctx->device);
     axr_server* s = new (std::nothrow) axr_server();
@@ -306,6 +318,7 @@ int axr_server_create(axr_ctx* ctx, const axr_server_config* cfg_in, axr_server*
     cc.max_views = cfg.max_batch_views;
     cc.max_faces = cfg.max_batch_faces;
     cc.max_tiles = cfg.max_batch_tiles;
+    cc.max_pixels = cfg.max_batch_pixels;
     cc.budget = cfg.hbm_budget;
     s->core.reset(new (std::nothrow) ServerCore<ServeReq>(cc, s->dev.get()));
     if (!s->core) { delete s; return AXR_ERR_OOM; }
@@ -336,11 +349,11 @@ const char* axr_server_last_error(axr_server* s) {
 
 int axr_server_submit(axr_server* s, const axr_serve_request* q, uint64_t* ticket) {
     if (!s || !ticket) return AXR_ERR_INVALID;
-    uint64_t bytes, faces, tiles;
-    int r = check_request(s, q, &bytes, &faces, &tiles);
+    uint64_t bytes, faces, tiles, pixels;
+    int r = check_request(s, q, &bytes, &faces, &tiles, &pixels);
     if (r) return r;
     ServeReq pl{q->mesh, q->params, q->width, q->height};
-    r = s->core->submit(pl, bytes, faces, tiles, now_s(), false, nullptr, ticket);
+    r = s->core->submit(pl, bytes, faces, tiles, pixels, now_s(), false, nullptr, ticket);
     if (r) return server_fail(s, r, s->core->last_error());
     return AXR_OK;
 }
@@ -423,15 +436,16 @@ int axr_server_replay(axr_server* s, uint32_t n, const axr_serve_request* reqs, 
                       double* latency_s) {
     if (!s || (n && (!reqs || !arrival_s || !latency_s))) return AXR_ERR_INVALID;
     std::vector<ServeReq> pls(n);
-    std::vector<uint64_t> bytes(n), faces(n), tiles(n);
+    std::vector<uint64_t> bytes(n), faces(n), tiles(n), pixels(n);
     for (uint32_t i = 0; i < n; ++i) {
-        int r = check_request(s, &reqs[i], &bytes[i], &faces[i], &tiles[i]);
+        int r = check_request(s, &reqs[i], &bytes[i], &faces[i], &tiles[i], &pixels[i]);
         if (r) return r;
         if (i && arrival_s[i] < arrival_s[i - 1])
             return server_fail(s, AXR_ERR_INVALID, "replay arrival times must be non-decreasing");
         pls[i] = ServeReq{reqs[i].mesh, reqs[i].params, reqs[i].width, reqs[i].height};
     }
-    const int r = s->core->replay(n, pls.data(), bytes.data(), faces.data(), tiles.data(), arrival_s, latency_s);
+    const int r = s->core->replay(n, pls.data(), bytes.data(), faces.data(), tiles.data(), pixels.data(), arrival_s,
+                                  latency_s);
     if (r) return server_fail(s, r, s->core->last_error());
     return AXR_OK;
 }

@@ -58,7 +58,7 @@ inline int class_of(uint64_t bytes) {
 template <class Payload>
 struct Job {
     Payload pl{};
-    uint64_t fb_bytes = 0, faces = 0, tiles = 0;
+    uint64_t fb_bytes = 0, faces = 0, tiles = 0, pixels = 0;
     Block fb;
     int state = JOB_FREE;
     int status = S_OK;
@@ -92,6 +92,7 @@ struct CoreConfig {
     uint32_t max_views = 64;
     uint64_t max_faces = 8ull << 20;
     uint64_t max_tiles = 1ull << 20;
+    uint64_t max_pixels = 0;   // framebuffer pixels per batch; 0 = unlimited
     uint64_t budget = 0;   // framebuffers + fixed bytes; 0 = unlimited
 };
 
@@ -154,7 +155,7 @@ public:
         }
     }
 
-    int submit(const Payload& pl, uint64_t fb_bytes, uint64_t faces, uint64_t tiles, double arrival,
+    int submit(const Payload& pl, uint64_t fb_bytes, uint64_t faces, uint64_t tiles, uint64_t pixels, double arrival,
                bool auto_release, double* out_latency, uint64_t* ticket, uint32_t* out_left = nullptr) {
         const int cls = class_of(fb_bytes);
         std::unique_lock<std::mutex> lk(mu_);
@@ -180,6 +181,7 @@ public:
         j.fb_bytes = fb_bytes;
         j.faces = faces;
         j.tiles = tiles;
+        j.pixels = pixels;
         j.state = JOB_QUEUED;
         j.auto_release = auto_release;
         j.t_arrival = arrival;
@@ -268,8 +270,9 @@ public:
     // calling thread spins on the clock), released at completion, and its
     // latency (completion - scheduled arrival) is written to latency_s[i]
     // (negative for a failed request).  Returns when all are done.
+    // pixels may be null (no pixel sizes: the max_pixels cap does not apply).
     int replay(uint32_t n, const Payload* pls, const uint64_t* fb_bytes, const uint64_t* faces,
-               const uint64_t* tiles, const double* arrival_s, double* latency_s) {
+               const uint64_t* tiles, const uint64_t* pixels, const double* arrival_s, double* latency_s) {
         for (uint32_t i = 0; i < n; ++i) latency_s[i] = -1.0;
         uint32_t left = n;   // guarded by mu_
         const double t0 = now_s();
@@ -277,7 +280,8 @@ public:
         for (uint32_t i = 0; i < n; ++i) {
             const double t = t0 + arrival_s[i];
             while (now_s() < t) std::this_thread::yield();
-            int r = submit(pls[i], fb_bytes[i], faces[i], tiles[i], t, true, &latency_s[i], nullptr, &left);
+            int r = submit(pls[i], fb_bytes[i], faces[i], tiles[i], pixels ? pixels[i] : 0, t, true, &latency_s[i],
+                           nullptr, &left);
             if (r != S_OK) {
                 rc = r;
                 std::lock_guard<std::mutex> lk(mu_);
@@ -400,14 +404,18 @@ private:
             Lane& lane = lanes_[l];
             if (lane.busy) continue;
             batch.clear();
-            uint64_t faces = 0, tiles = 0;
+            uint64_t faces = 0, tiles = 0, pixels = 0;
             bool deferred = false;
             while (!queue_.empty() && batch.size() < cfg_.max_views) {
                 JobT& j = jobs_[queue_.front()];
-                if (!batch.empty() && (faces + j.faces > cfg_.max_faces || tiles + j.tiles > cfg_.max_tiles)) break;
+                // a single request above a cap runs alone
+                if (!batch.empty() && (faces + j.faces > cfg_.max_faces || tiles + j.tiles > cfg_.max_tiles ||
+                                       (cfg_.max_pixels && pixels + j.pixels > cfg_.max_pixels)))
+                    break;
                 if (!take_block(j.fb_bytes, &j.fb)) { deferred = true; break; }
                 faces += j.faces;
                 tiles += j.tiles;
+                pixels += j.pixels;
                 lane.slots.push_back(queue_.front());
                 batch.push_back(&j);
                 queue_.pop_front();

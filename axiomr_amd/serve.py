@@ -39,7 +39,7 @@ def request(mesh: Mesh, width: int, height: int, params: N.AxrDrawParams) -> N.A
 class Server:
     def __init__(self, ctx: Context, lanes: int = 3, max_batch_views: int = 64, max_batch_faces: int = 8 << 20,
                  max_batch_tiles: int = 1 << 20, hbm_budget: int = 0, clear: Optional[Color] = None,
-                 with_ids: bool = False):
+                 with_ids: bool = False, max_batch_pixels: int = 0):
         self.ctx = ctx
         self._lib = ctx._lib
         cfg = N.AxrServerConfig()
@@ -47,6 +47,7 @@ class Server:
         cfg.max_batch_views = max_batch_views
         cfg.max_batch_faces = max_batch_faces
         cfg.max_batch_tiles = max_batch_tiles
+        cfg.max_batch_pixels = max_batch_pixels
         cfg.hbm_budget = hbm_budget
         cfg.clear_bgra = (clear or Color(0, 0, 0, 255)).packed_bgra()
         cfg.with_ids = 1 if with_ids else 0

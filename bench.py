@@ -98,21 +98,76 @@ def warm(ctx, enqueue, tries=8):
 
 
 def stage_names(cfg, two_pass=False):
-    """Names of the five event-timed stages in the context's raster mode;
+    """Kernel names of the five timed stages in the context's raster mode;
     two_pass: visibility-mode draws that resolve in two launches (an
-    asynchronous draw with a clear into a framebuffer without ids)."""
+    asynchronous draw with a clear)."""
     if cfg.get("raster_mode", 0) == 1:
-        return ("vis", "clip", "big", "shade", "finish") if two_pass else ("vis", "clip", "big", "none", "resolve")
-    return ("setup", "clip", "scan", "scatter", "raster")
+        return ("k_vis", "k_clip", "k_big", "k_shade", "k_finish") if two_pass else \
+            ("k_vis", "k_clip", "k_big", "none", "k_resolve")
+    return ("k_setup", "k_clip", "k_scan", "k_scatter", "k_raster")
 
 
-def run_plain(scene, steps, warmup, tile, use_tuned):
+def stage_times(ctx, cfg, two_pass=False):
+    """Average kernel time (ms) of each stage over the frames timed since
+    profiling was switched on: HIP events attached to the kernels' own
+    dispatches (hipExtLaunchKernel), so they read what a kernel trace reads."""
+    st = ctx.stats()
+    n = max(1, st.frames)
+    return {name: st.stage_ms_total[i] / n for i, name in enumerate(stage_names(cfg, two_pass)) if name != "none"}
+
+
+def compare_frames(gpu, ref, gpu_ids=None):
+    """Parity of a GPU frame (bgra, depth) against the checker's frame
+    (bgra, depth, ids): coverage and depth bits must match exactly, colour
+    to within 1 LSB per channel.  gpu_ids: the winner ids of a frame drawn
+    by the same kernels into a framebuffer with an id plane."""
+    gc, gd = gpu[0], gpu[1]
+    rc, rd, ri = ref
+    inf = np.float32(np.inf).view(np.uint32)
+    cov_g, cov_r = gd.view(np.uint32) != inf, rd.view(np.uint32) != inf
+    diff = np.abs(gc.astype(np.int16) - rc.astype(np.int16))
+    rep = {"pixels": int(gd.size), "covered": int(cov_r.sum()),
+           "coverage_mismatch": int((cov_g != cov_r).sum()),
+           "depth_mismatch": int((gd.view(np.uint32) != rd.view(np.uint32)).sum()),
+           "winner_mismatch": int((gpu_ids != ri).sum()) if gpu_ids is not None else None,
+           "color_max": int(diff.max()) if diff.size else 0,
+           "color_mean": float(diff.mean()) if diff.size else 0.0}
+    return rep
+
+
+def parity_ok(rep):
+    return (rep["coverage_mismatch"] == 0 and rep["depth_mismatch"] == 0 and not rep.get("winner_mismatch")
+            and rep["color_max"] <= 1)
+
+
+def merge_parity(acc, rep):
+    """Sum of per-view parity reports (colour max is the max, mean is
+    pixel-weighted)."""
+    if acc is None:
+        return dict(rep)
+    out = dict(acc)
+    for k in ("pixels", "covered", "coverage_mismatch", "depth_mismatch"):
+        out[k] = acc[k] + rep[k]
+    if acc.get("winner_mismatch") is not None and rep.get("winner_mismatch") is not None:
+        out["winner_mismatch"] = acc["winner_mismatch"] + rep["winner_mismatch"]
+    out["color_max"] = max(acc["color_max"], rep["color_max"])
+    out["color_mean"] = (acc["color_mean"] * acc["pixels"] + rep["color_mean"] * rep["pixels"]) / out["pixels"]
+    return out
+
+
+def run_frames(scene, steps, warmup, tile, use_tuned, workload="plain"):
+    """The plain per-frame GPU path on one scene: `steps` frames of
+    clear + asynchronous drawMesh into one framebuffer, timed between two
+    stream synchronisations; the last timed frame is read back.  Then as
+    many frames again with events on each kernel's dispatch give the
+    kernels' durations, and one more frame through the same kernels into a
+    framebuffer with an id plane gives the winners."""
     import axiomr_amd as axr
     from axiomr_amd import render as R
     from axiomr_amd import resident as H
 
     ctx = axr.Context(0)
-    cfg = apply_config(ctx, "plain", tile, use_tuned)
+    cfg = apply_config(ctx, workload, tile, use_tuned)
     gs = H.GpuScene(ctx, scene)
     fb = R.Framebuffer(ctx, scene.width, scene.height)
     params = H.params_of(scene, flags=1)   # AXR_DRAW_ASYNC
@@ -121,9 +176,9 @@ def run_plain(scene, steps, warmup, tile, use_tuned):
     inf = float("inf")
     pp = C.byref(params)
 
-    def frame():
-        L.axr_fb_clear(fb.handle, bgra, inf)
-        L.axr_draw(ctx.handle, fb.handle, gs.mesh.handle, pp)
+    def frame(target=fb):
+        L.axr_fb_clear(target.handle, bgra, inf)
+        L.axr_draw(ctx.handle, target.handle, gs.mesh.handle, pp)
 
     # first frame synchronously: sizes the frame arena, gives coverage
     fb.clear(H.BLACK)
@@ -138,29 +193,45 @@ def run_plain(scene, steps, warmup, tile, use_tuned):
         frame()
     ctx.sync()
     dt = time.perf_counter() - t0
-
-    # profiling pass: HIP events between the stages, same frames
+    last = fb.read()
+    # kernel times: the same frames again with events on each kernel's
+    # dispatch (a dispatch that carries events runs slower end to end, so
+    # these frames are not the ones `value` times)
     ctx.set_profiling(True)
-    prof_steps = max(10, min(steps, 200))
-    for _ in range(prof_steps):
+    for _ in range(steps):
         frame()
     ctx.sync()
-    st = ctx.stats()
+    stages = stage_times(ctx, cfg, two_pass=True)
     ctx.set_profiling(False)
-    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(stage_names(cfg, True))}
-    frame_ms = st.frame_ms_total / max(1, st.frames)
+    fbi = R.Framebuffer(ctx, scene.width, scene.height, with_ids=True)
+    frame(fbi)
+    ctx.sync()
+    checked = fbi.read()
+    fbi.close()
     in_use, arena = ctx.memory()
     fb.close()
     gs.close()
     ctx.close()
-    return dict(seconds=dt, covered=covered, stages_ms=stages, frame_event_ms=frame_ms, mem_in_use=in_use,
-                arena_bytes=arena, config=cfg)
+    return dict(seconds=dt, covered=covered, stages_ms=stages, mem_in_use=in_use, arena_bytes=arena, config=cfg,
+                last=last, checked=checked)
+
+
+def frame_kernel_bytes(nverts, nfaces, W, H, covered, ntex):
+    """Algorithmic bytes of each kernel of the plain visibility-mode frame
+    (they sum to the frame's B = 12 V + 12 T + 8 W H + 4 N_tex P_cov): k_vis
+    reads positions and indices, k_shade one texel per texture per covered
+    pixel, k_finish writes colour and depth of every pixel (the absorbed
+    clear included); the clipper and large-triangle passes have ~none."""
+    return {"k_vis": setup_bytes(nverts, nfaces), "k_clip": 0, "k_big": 0, "k_shade": 4 * ntex * covered,
+            "k_finish": 8 * W * H, "k_resolve": resolve_bytes(W, H, covered, ntex)}
 
 
 def run_batch(wl, reps, tile, use_tuned):
     """The batched form (256 views, 32 meshes) against the plain per-frame
-    path on the same views: frames (views) per second of each.  Both use the
-    same tunables (the batch's tuned ones when cached)."""
+    path on the same views: frames (views) per second of each.  Returns the
+    measurements plus the live context and batch (`gb`), whose framebuffers
+    hold the batch's last frames for the parity check; the caller closes
+    both."""
     import axiomr_amd as axr
     from axiomr_amd import resident as H
 
@@ -202,18 +273,34 @@ def run_batch(wl, reps, tile, use_tuned):
         gb.batch.launch()
     ctx.sync()
     batch_s = (time.perf_counter() - t0) / reps
+    # kernel times: the same launch sequence outside the graph, with events
+    # on each kernel's dispatch
     ctx.set_profiling(True)
-    gb.batch.launch()
+    for _ in range(3):
+        gb.batch.launch()
     ctx.sync()
-    st = ctx.stats()
+    stages = stage_times(ctx, cfg)
     ctx.set_profiling(False)
-    stages = {name: st.stage_ms_total[i] / max(1, st.frames) for i, name in enumerate(stage_names(cfg))}
-    gb.close()
-    ctx.close()
+    gb.batch.launch()   # the frames the parity check reads: drawn by the graph
+    ctx.sync()
+    covered = 0
+    for fbv in gb.fbs:
+        _, d, _ = fbv.read()
+        covered += int(np.isfinite(d).sum())
     return dict(views=n, plain_views_per_s=n / plain_s, batch_views_per_s=n / batch_s, speedup=plain_s / batch_s,
                 sync_views_per_s=n / sync_s, speedup_vs_sync=sync_s / batch_s,
-                batch_ms=batch_s * 1e3, plain_ms=plain_s * 1e3, stages_ms=stages,
-                faces=wl.total_faces, pixels=wl.total_pixels, config=cfg, plain_config=plain_cfg)
+                batch_ms=batch_s * 1e3, plain_ms=plain_s * 1e3, stages_ms=stages, covered=covered,
+                faces=wl.total_faces, pixels=wl.total_pixels, config=cfg, plain_config=plain_cfg, gb=gb, ctx=ctx)
+
+
+def batch_kernel_bytes(wl, covered, ntex=2):
+    """Algorithmic bytes per launch of the batch's kernels: k_setup reads
+    every view's positions and indices (12 V + 12 T), k_raster writes colour
+    and depth of the covered pixels and reads one texel per texture per
+    covered pixel (the clear pass writes the rest of every view)."""
+    verts = sum(wl.meshes[v.mesh][0].shape[0] for v in wl.views)
+    return {"k_setup": 12 * verts + 12 * wl.total_faces, "k_raster": (8 + 4 * ntex) * covered,
+            "k_clear_views": 8 * wl.total_pixels}
 
 
 SERVE_RATES = (100, 250, 500, 1000, 2500, 5000)
@@ -319,28 +406,35 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_batch(wl, threads=16):
+def cpu_batch(wl, gb=None, threads=16):
     """The CPU checker over every view of the batch, one after another
-    (one frame at a time, its thread pool inside each frame)."""
+    (one frame at a time, its thread pool inside each frame).  gb: the GPU
+    batch whose framebuffers hold the batch's frames; each view is compared
+    with the checker's frame (coverage, depth, colour), outside the timing."""
     import oracle as O
     from axiomr_amd import resident as H
     threads = min(threads, os.cpu_count() or 1)
     rr = O.OracleRenderer(threads)
     oscenes = {}
-    t0 = time.perf_counter()
+    dt = 0.0
+    par = None
     for i, vs in enumerate(wl.views):
         sc = wl.scene(i)
         if vs.mesh not in oscenes:
             oscenes[vs.mesh] = O.OracleScene(sc)
         fb = rr.framebuffer(sc.width, sc.height)
+        t0 = time.perf_counter()
         O.fb_clear(fb, H.BLACK.packed_bgra(), float("inf"))
         rr.draw(fb, oscenes[vs.mesh], H.params_of(sc))
+        dt += time.perf_counter() - t0
+        if gb is not None:
+            ref = O.fb_read(fb, sc.width, sc.height)
+            par = merge_parity(par, compare_frames(gb.fbs[i].read(), ref))
         O.fb_destroy(fb)
-    dt = time.perf_counter() - t0
     for o in oscenes.values():
         o.close()
     rr.close()
-    return dict(views_per_s=len(wl.views) / dt, seconds=dt, threads=threads)
+    return dict(views_per_s=len(wl.views) / dt, seconds=dt, threads=threads, parity=par)
 
 
 def cpu_serve(wl, n=24, threads=16):
@@ -370,9 +464,11 @@ def cpu_serve(wl, n=24, threads=16):
                 p99_ms=float(np.percentile(t, 99) * 1e3), frames_per_s=float(1.0 / t.mean()), threads=threads)
 
 
-def cpu_baseline(scene, budget_s=12.0, threads=16):
+def cpu_baseline(scene, budget_s=12.0, threads=16, min_frames=1, max_frames=50):
     """The CPU checker (restated reference tiled pipeline with its thread
-    pool) on the same frame, repeated until ~budget_s of CPU work."""
+    pool) on the same frame, repeated until ~budget_s of CPU work (at least
+    min_frames, at most max_frames frames); median frame time.  Also returns
+    the last frame (bgra, depth, ids) for the parity check."""
     import oracle as O
     from axiomr_amd import resident as H
     threads = min(threads, os.cpu_count() or 1)
@@ -387,14 +483,15 @@ def cpu_baseline(scene, budget_s=12.0, threads=16):
         t0 = time.perf_counter()
         rr.draw(fb, osc, p)
         times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > budget_s or len(times) >= 50:
+        if len(times) >= max_frames or (len(times) >= min_frames and time.perf_counter() - t_start > budget_s):
             break
     phases = rr.phase_times()
+    ref = O.fb_read(fb, scene.width, scene.height)
     O.fb_destroy(fb)
     rr.close()
     osc.close()
     med = float(np.median(times))
-    return dict(value=1.0 / med, frames=len(times), median_s=med, threads=threads,
+    return dict(value=1.0 / med, frames=len(times), median_s=med, threads=threads, frame=ref,
                 phases_s={k: v / len(times) for k, v in phases.items()})
 
 
@@ -440,6 +537,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 800x600 UV-sphere and 4M-triangle soup legs")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-form leg")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="stream length per arrival rate")
@@ -468,35 +567,76 @@ def main():
     log(f"[bench] scene {scene.name}: {scene.nfaces} faces, {scene.vertices.shape[0]} vertices, "
         f"checksum {scene.checksum()} ({time.perf_counter() - t:.1f}s)")
 
-    r = run_plain(scene, args.steps, args.warmup, args.tile, not args.untuned)
+    r = run_frames(scene, args.steps, args.warmup, args.tile, not args.untuned)
     fps = args.steps / r["seconds"]
     ms = 1000.0 * r["seconds"] / args.steps
     nverts, nfaces = scene.vertices.shape[0], scene.nfaces
     W, H = scene.width, scene.height
     ntex = 2
-    log(f"[bench] plain path: {fps:.1f} frames/s ({ms:.3f} ms/frame), stages {r['stages_ms']}")
+    log(f"[bench] plain path: {fps:.1f} frames/s ({ms:.4f} ms/frame), kernel ms {r['stages_ms']}")
 
-    stages = dict(r["stages_ms"])
-    if "shade" in stages:
-        # the resolve runs as two launches, k_shade then k_finish: priced
-        # together, as the one k_resolve they replace
-        stages["resolve"] = stages["shade"] + stages["finish"]
-        for k in ("shade", "finish"):
-            stages.pop(k)
-    dominant = max(stages, key=stages.get)
-    if dominant in ("setup", "vis"):
-        kbytes, kname = setup_bytes(nverts, nfaces), "k_" + dominant
-    elif "resolve" in stages:
-        dominant = "resolve"
-        kbytes = resolve_bytes(W, H, r["covered"], ntex)
-        # k_finish4 (16-byte writes) when the width is a multiple of 4
-        kname = ("k_shade+" + ("k_finish4" if W % 4 == 0 else "k_finish")) if "shade" in r["stages_ms"] else "k_resolve"
-    else:
-        dominant = "raster"
-        kbytes, kname = raster_bytes(r["covered"], ntex), "k_raster"
-    kms = stages[dominant]
+    # roofline: each kernel's algorithmic bytes over its average duration
+    # (events on its dispatches in the timed frames); the dominant kernel is
+    # the one with the longest average duration
+    kb = frame_kernel_bytes(nverts, nfaces, W, H, r["covered"], ntex)
+    kernels = {}
+    for name, kms_ in r["stages_ms"].items():
+        nb = kb.get(name, 0)
+        kernels[name] = {"avg_ms": round(kms_, 6), "bytes_per_launch": nb,
+                         "achieved_gbs": round(nb / (kms_ * 1e-3) / 1e9, 2) if kms_ > 0 else None,
+                         "frac": round(nb / (kms_ * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) if kms_ > 0 else None}
+    dominant = max(r["stages_ms"], key=r["stages_ms"].get)
+    kms = r["stages_ms"][dominant]
+    kbytes = kb.get(dominant, 0)
+    # k_finish4 (16-byte writes) when the width is a multiple of 4
+    kname = "k_finish4" if dominant == "k_finish" and W % 4 == 0 else dominant
     achieved = kbytes / (kms * 1e-3) / 1e9
     frame_bytes = algorithmic_bytes(nverts, nfaces, W, H, r["covered"], ntex)
+
+    cpu, parity = None, None
+    extra_configs = {}
+    if not args.no_cpu:
+        c = cpu_baseline(scene, args.cpu_budget)
+        parity = compare_frames(r["last"], c["frame"], gpu_ids=r["checked"][2])
+        # the check frame with ids must be the timed frame again
+        parity["check_frame_equal"] = bool(np.array_equal(r["checked"][0], r["last"][0]) and
+                                           np.array_equal(r["checked"][1].view(np.uint32),
+                                                          r["last"][1].view(np.uint32)))
+        log(f"[bench] parity of the last timed frame against the checker: {parity}")
+        log(f"[bench] cpu baseline: {c['value']:.3f} frames/s over {c['frames']} frames, phases {c['phases_s']}")
+        c1 = cpu_baseline(scene, args.cpu_budget / 2, threads=1)
+        log(f"[bench] cpu baseline, 1 thread: {c1['value']:.3f} frames/s over {c1['frames']} frames")
+        cpu = {"value": round(c["value"], 4), "unit": "frames/s", "cores": c["threads"], "kind": "port",
+               "sample": f"{c['frames']} frames of the same 1080p 1M-triangle scene, median",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+               "single_thread_value": round(c1["value"], 4), "single_thread_frames": c1["frames"],
+               "phases_s": {k: round(v, 5) for k, v in c["phases_s"].items()}}
+
+    if not args.no_configs:
+        # the reference's own CPU case (800x600 UV sphere, Gouraud x checker)
+        # and the overdraw/tie stress (4M-triangle soup): GPU frames/s and
+        # Gtri/s, the checker's frames/s, parity of the last timed frame
+        for key, mk, min_f, max_f in (("uv_sphere_800x600", lambda: S.uv_sphere_scene(), 50, 200),
+                                      ("soup_4M_1080p", lambda: S.soup_scene(), 5, 5)):
+            t = time.perf_counter()
+            sc = mk()
+            log(f"[bench] scene {sc.name}: {sc.nfaces} faces ({time.perf_counter() - t:.1f}s)")
+            rr_ = run_frames(sc, args.steps, args.warmup, args.tile, not args.untuned)
+            gfps = args.steps / rr_["seconds"]
+            rec = {"workload": sc.name, "triangles": sc.nfaces, "width": sc.width, "height": sc.height,
+                   "frames_per_s": round(gfps, 2), "gtri_per_s": round(gfps * sc.nfaces / 1e9, 4),
+                   "ms_per_frame": round(1000.0 * rr_["seconds"] / args.steps, 5),
+                   "kernel_ms": {k: round(v, 6) for k, v in rr_["stages_ms"].items()},
+                   "covered_pixels": rr_["covered"]}
+            if not args.no_cpu:
+                cc = cpu_baseline(sc, args.cpu_budget, min_frames=min_f, max_frames=max_f)
+                rec["cpu_frames_per_s"] = round(cc["value"], 4)
+                rec["cpu_frames"] = cc["frames"]
+                rec["cpu_threads"] = cc["threads"]
+                rec["cpu_phases_s"] = {k: round(v, 5) for k, v in cc["phases_s"].items()}
+                rec["parity"] = compare_frames(rr_["last"], cc["frame"], gpu_ids=rr_["checked"][2])
+            log(f"[bench] {key}: {rec}")
+            extra_configs[key] = rec
 
     batch = None
     if not args.no_batch:
@@ -505,25 +645,38 @@ def main():
         log(f"[bench] batch workload: {len(wl.views)} views, {wl.total_faces} faces, {wl.total_pixels} pixels "
             f"({time.perf_counter() - t:.1f}s)")
         b = run_batch(wl, args.batch_reps, args.tile, not args.untuned)
-        cb = None if args.no_cpu else cpu_batch(wl)
+        cb = None if args.no_cpu else cpu_batch(wl, b["gb"])
+        b["gb"].close()
+        b["ctx"].close()
         if cb:
-            log(f"[bench] batch on the CPU checker: {cb['views_per_s']:.1f} views/s ({cb['threads']} threads)")
+            log(f"[bench] batch on the CPU checker: {cb['views_per_s']:.1f} views/s ({cb['threads']} threads), "
+                f"parity {cb['parity']}")
         log(f"[bench] batch: plain {b['plain_views_per_s']:.0f} views/s (queued), {b['sync_views_per_s']:.0f} views/s "
             f"(synchronous drawMesh), batched {b['batch_views_per_s']:.0f} views/s, x{b['speedup']:.2f} / "
-            f"x{b['speedup_vs_sync']:.2f}; stages {b['stages_ms']}")
+            f"x{b['speedup_vs_sync']:.2f}; kernel ms {b['stages_ms']}")
+        bkb = batch_kernel_bytes(wl, b["covered"], ntex)
+        bdom = max(b["stages_ms"], key=b["stages_ms"].get)
+        bms = b["stages_ms"][bdom]
+        bach = bkb.get(bdom, 0) / (bms * 1e-3) / 1e9 if bms > 0 else 0.0
         batch = {"workload": f"{len(wl.views)} views of 32 displaced icospheres (5k-500k triangles, log-uniform), "
                              "256x256..1920x1080 log-uniform, Phong + textures, one graph launch per batch",
-                 "views": b["views"], "triangles": b["faces"], "pixels": b["pixels"],
+                 "views": b["views"], "triangles": b["faces"], "pixels": b["pixels"], "covered_pixels": b["covered"],
                  "batched_views_per_s": round(b["batch_views_per_s"], 1),
                  "plain_views_per_s": round(b["plain_views_per_s"], 1),
                  "speedup_vs_plain": round(b["speedup"], 3),
                  "sync_views_per_s": round(b["sync_views_per_s"], 1),
                  "speedup_vs_sync": round(b["speedup_vs_sync"], 3),
                  "batched_gtri_per_s": round(b["faces"] / (b["batch_ms"] * 1e-3) / 1e9, 3),
-                 "batch_ms": round(b["batch_ms"], 3), "stages_ms": {k: round(v, 4) for k, v in b["stages_ms"].items()},
+                 "batch_ms": round(b["batch_ms"], 4), "kernel_ms": {k: round(v, 5) for k, v in b["stages_ms"].items()},
+                 "roofline": {"bound": "hbm", "kernel": bdom, "achieved": round(bach, 2), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(bach / HBM_PEAK_GBS, 5),
+                              "bytes_per_launch": bkb.get(bdom, 0), "avg_launch_ms": round(bms, 5)},
+                 "batch_algorithmic_bytes": sum(bkb.values()),
+                 "batch_roofline_frac": round(sum(bkb.values()) / (b["batch_ms"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 5),
                  "tunables": b["config"], "plain_tunables": b["plain_config"],
                  "cpu_views_per_s": round(cb["views_per_s"], 2) if cb else None,
-                 "cpu_threads": cb["threads"] if cb else None}
+                 "cpu_threads": cb["threads"] if cb else None,
+                 "parity": cb["parity"] if cb else None}
 
     served = None
     if not args.no_serve:
@@ -555,18 +708,6 @@ def main():
                                       "p99": round(cs["p99_ms"], 2), "requests": cs["requests"],
                                       "threads": cs["threads"]} if cs else None)}
 
-    cpu = None
-    if not args.no_cpu:
-        c = cpu_baseline(scene, args.cpu_budget)
-        log(f"[bench] cpu baseline: {c['value']:.3f} frames/s over {c['frames']} frames, phases {c['phases_s']}")
-        c1 = cpu_baseline(scene, args.cpu_budget / 2, threads=1)
-        log(f"[bench] cpu baseline, 1 thread: {c1['value']:.3f} frames/s over {c1['frames']} frames")
-        cpu = {"value": round(c["value"], 4), "unit": "frames/s", "cores": c["threads"], "kind": "port",
-               "sample": f"{c['frames']} frames of the same 1080p 1M-triangle scene, median",
-               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
-               "single_thread_value": round(c1["value"], 4), "single_thread_frames": c1["frames"],
-               "phases_s": {k: round(v, 5) for k, v in c["phases_s"].items()}}
-
     out = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -587,16 +728,30 @@ def main():
         "gtri_per_s": round(fps * nfaces / 1e9, 4),
         "frame_algorithmic_bytes": frame_bytes,
         "frame_roofline_frac": round(frame_bytes * fps / (HBM_PEAK_GBS * 1e9), 5),
-        "stages_ms": {k: round(v, 5) for k, v in stages.items()},
+        "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic(kname, r["config"]),
-                     "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 5)},
+                     "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 6)},
         "cpu_baseline": cpu,
+        "parity": parity,
+        "configs": extra_configs or None,
         "batched": batch,
         "served": served,
     }
     print(json.dumps(out), flush=True)
+    # a fast frame whose pixels differ from the reference's is not a result
+    bad = []
+    if parity is not None and not (parity_ok(parity) and parity["check_frame_equal"]):
+        bad.append(f"1080p frame {parity}")
+    for key, rec in extra_configs.items():
+        if rec.get("parity") is not None and not parity_ok(rec["parity"]):
+            bad.append(f"{key} {rec['parity']}")
+    if batch and batch.get("parity") is not None and not parity_ok(batch["parity"]):
+        bad.append(f"batch {batch['parity']}")
+    if bad:
+        log("[bench] PARITY FAILED: " + "; ".join(bad))
+        sys.exit(1)
 
 
 if __name__ == "__main__":

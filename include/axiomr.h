@@ -183,8 +183,23 @@ int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids);
 /* reference TiledPipeline::drawMesh (src/tiled_pipeline.cpp:143-322). */
 int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_params* params);
 
+/* Size limits of one view (one drawMesh), checked by axr_draw, batch
+ * creation and server submission: AXR_OK, or AXR_ERR_INVALID when the view
+ * submits 2^29 faces or more (the depth tie-break key, submission rank * 8
+ * + clip sub-triangle, is 32 bits; reference tiled_pipeline.cpp:249-258
+ * orders by the same (group, face, sub-triangle) triple) or the
+ * framebuffer is not 1..32768 pixels on each edge.  Host only: no device
+ * is needed. */
+int axr_check_view(uint64_t submitted_faces, int width, int height);
+
 /* ---- batched form: many views in one graph-captured launch sequence ---- */
-/* Each view i draws meshes[i] into fbs[i] with params[i]. */
+/* Each view i draws meshes[i] into fbs[i] with params[i].  Views have no
+ * order among themselves, so each framebuffer may appear in one view only
+ * (AXR_ERR_INVALID otherwise); several meshes into one framebuffer, as the
+ * reference's Renderer::render loop draws them (src/renderer.cpp:100-110),
+ * are consecutive axr_draw calls or one batch per mesh.  The batch holds
+ * its views' framebuffers and meshes: axr_fb_destroy and axr_mesh_destroy
+ * fail with AXR_ERR_INVALID on them until the batch is destroyed. */
 int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const axr_mesh* const* meshes,
                      const axr_draw_params* params, axr_batch** out);
 /* Clear every view's framebuffer (bgra packed as in axr_fb_clear, depth)
@@ -212,6 +227,10 @@ typedef struct axr_server_config {
                                    0 = no limit beyond the context's */
     uint32_t clear_bgra;        /* clear colour of every request */
     int with_ids;               /* keep each request's winner-key plane */
+    uint64_t max_batch_pixels;  /* framebuffer pixels per batch; a larger single
+                                   request runs alone (0: 32 Mi).  Each lane's
+                                   frame arena is sized for max_batch_faces
+                                   faces over this many pixels up front */
 } axr_server_config;
 
 typedef struct axr_serve_request {

@@ -95,7 +95,7 @@ static void test_fifo_and_batching() {
     for (int i = 0; i < 60; ++i) {
         Req r;
         r.id = i;
-        CHECK(core.submit(r, 2 * MB, 300, 100, now_s(), false, nullptr, &t[i]) == S_OK);
+        CHECK(core.submit(r, 2 * MB, 300, 100, 0, now_s(), false, nullptr, &t[i]) == S_OK);
     }
     for (int i = 0; i < 60; ++i) CHECK(core.wait(t[i], 10.0) == S_OK);
     core.stop();
@@ -122,11 +122,11 @@ static void test_oversized_single_request_runs_alone() {
     uint64_t a, b, c;
     Req r;
     r.id = 0;
-    CHECK(core.submit(r, MB, 5000, 1, now_s(), false, nullptr, &a) == S_OK);
+    CHECK(core.submit(r, MB, 5000, 1, 0, now_s(), false, nullptr, &a) == S_OK);
     r.id = 1;
-    CHECK(core.submit(r, MB, 10, 1, now_s(), false, nullptr, &b) == S_OK);
+    CHECK(core.submit(r, MB, 10, 1, 0, now_s(), false, nullptr, &b) == S_OK);
     r.id = 2;
-    CHECK(core.submit(r, MB, 10, 1, now_s(), false, nullptr, &c) == S_OK);
+    CHECK(core.submit(r, MB, 10, 1, 0, now_s(), false, nullptr, &c) == S_OK);
     CHECK(core.wait(a, 10) == S_OK && core.wait(b, 10) == S_OK && core.wait(c, 10) == S_OK);
     core.stop();
     for (auto& bt : dev.batches)
@@ -134,6 +134,33 @@ static void test_oversized_single_request_runs_alone() {
             if (id == 0) CHECK(bt.size() == 1);
     core.release_pool();
     std::puts("oversized_single ok");
+}
+
+static void test_pixel_cap() {
+    // batches hold at most max_pixels framebuffer pixels; a single larger
+    // request runs alone
+    FakeDevice dev(1, 0, 0.0005);
+    CoreConfig cc;
+    cc.max_views = 16;
+    cc.max_pixels = 1000;
+    ServerCore<Req> core(cc, &dev);
+    core.pause(true);   // queue everything first, so batching is deterministic
+    core.start();
+    std::vector<uint64_t> t(9);
+    const uint64_t px[9] = {400, 400, 400, 5000, 100, 100, 1000, 1, 999};
+    for (int i = 0; i < 9; ++i) {
+        Req r;
+        r.id = i;
+        CHECK(core.submit(r, MB, 1, 1, px[i], now_s(), false, nullptr, &t[i]) == S_OK);
+    }
+    core.pause(false);
+    for (int i = 0; i < 9; ++i) CHECK(core.wait(t[i], 10.0) == S_OK);
+    core.stop();
+    const std::vector<std::vector<int>> want = {{0, 1}, {2}, {3}, {4, 5}, {6}, {7, 8}};
+    CHECK(dev.batches == want);
+    for (int i = 0; i < 9; ++i) CHECK(core.release(t[i]) == S_OK);
+    core.release_pool();
+    std::puts("pixel_cap ok");
 }
 
 static void test_budget_defers_and_rejects() {
@@ -146,11 +173,11 @@ static void test_budget_defers_and_rejects() {
     core.start();
     uint64_t big;
     Req r;
-    CHECK(core.submit(r, 3 * blk, 1, 1, now_s(), false, nullptr, &big) == S_BUDGET);   // can never fit
+    CHECK(core.submit(r, 3 * blk, 1, 1, 0, now_s(), false, nullptr, &big) == S_BUDGET);   // can never fit
     std::vector<uint64_t> t(12);
     for (int i = 0; i < 12; ++i) {
         r.id = i;
-        CHECK(core.submit(r, 4 * MB, 1, 1, now_s(), false, nullptr, &t[i]) == S_OK);
+        CHECK(core.submit(r, 4 * MB, 1, 1, 0, now_s(), false, nullptr, &t[i]) == S_OK);
     }
     // the client releases results as they complete; the rest wait in the queue
     for (int i = 0; i < 12; ++i) {
@@ -180,10 +207,10 @@ static void test_eviction_of_idle_blocks() {
     ServerCore<Req> core(cc, &dev);
     Req r;
     uint64_t a, b;
-    CHECK(core.submit(r, 2 * MB, 1, 1, now_s(), false, nullptr, &a) == S_OK);
+    CHECK(core.submit(r, 2 * MB, 1, 1, 0, now_s(), false, nullptr, &a) == S_OK);
     while (core.state(a) != JOB_DONE) core.step();
     CHECK(core.release(a) == S_OK);
-    CHECK(core.submit(r, 16 * MB, 1, 1, now_s(), false, nullptr, &b) == S_OK);
+    CHECK(core.submit(r, 16 * MB, 1, 1, 0, now_s(), false, nullptr, &b) == S_OK);
     for (int k = 0; k < 100 && core.state(b) != JOB_DONE; ++k) core.step();
     CHECK(core.state(b) == JOB_DONE);
     CHECK(dev.frees == 1 && dev.live_bytes == large);
@@ -202,7 +229,7 @@ static void test_failures() {
         r.id = i;
         r.fail_launch = (i == 5);
         r.fail_run = (i == 11);
-        CHECK(core.submit(r, MB, 1, 1, now_s(), false, nullptr, &t[i]) == S_OK);
+        CHECK(core.submit(r, MB, 1, 1, 0, now_s(), false, nullptr, &t[i]) == S_OK);
     }
     int failed = 0;
     for (int i = 0; i < 20; ++i) {
@@ -229,7 +256,7 @@ static void test_release_rules() {
     ServerCore<Req> core(CoreConfig{}, &dev);
     Req r;
     uint64_t a;
-    CHECK(core.submit(r, MB, 1, 1, now_s(), false, nullptr, &a) == S_OK);
+    CHECK(core.submit(r, MB, 1, 1, 0, now_s(), false, nullptr, &a) == S_OK);
     CHECK(core.release(a) == S_NOT_READY);            // still queued
     CHECK(core.state(a) == JOB_QUEUED);
     core.step();
@@ -241,7 +268,7 @@ static void test_release_rules() {
     CHECK(core.state(a) == S_INVALID);
     CHECK(core.done_block(a, &b, &back) == S_INVALID);
     uint64_t a2;
-    CHECK(core.submit(r, MB, 1, 1, now_s(), false, nullptr, &a2) == S_OK);
+    CHECK(core.submit(r, MB, 1, 1, 0, now_s(), false, nullptr, &a2) == S_OK);
     CHECK((a2 & 0xFFFFFFFFu) == (a & 0xFFFFFFFFu) && a2 != a);   // slot reused, new generation
     CHECK(core.state(a) == S_INVALID);
     core.release_pool();
@@ -258,7 +285,7 @@ static void test_pause_batches_a_burst() {
     for (int i = 0; i < 10; ++i) {
         Req r;
         r.id = i;
-        CHECK(core.submit(r, MB, 1, 1, now_s(), false, nullptr, &t[i]) == S_OK);
+        CHECK(core.submit(r, MB, 1, 1, 0, now_s(), false, nullptr, &t[i]) == S_OK);
     }
     CHECK(!core.step() && core.queued() == 10 && dev.batches.empty());   // nothing admitted while paused
     core.pause(false);
@@ -286,7 +313,7 @@ static void test_replay_open_loop() {
         arr[i] = i * 0.0001;       // 10k requests per second: batches form behind busy lanes
         if (i % 7 == 0) bytes[i] = 9 * MB;
     }
-    CHECK(core.replay(n, pls.data(), bytes.data(), faces.data(), tiles.data(), arr.data(), lat.data()) == S_OK);
+    CHECK(core.replay(n, pls.data(), bytes.data(), faces.data(), tiles.data(), nullptr, arr.data(), lat.data()) == S_OK);
     core.stop();
     for (uint32_t i = 0; i < n; ++i) CHECK(lat[i] >= 0.0 && lat[i] < 5.0);
     CoreStats s = core.stats();
@@ -309,7 +336,7 @@ static void test_concurrent_clients() {
                 Req r;
                 r.id = c * 1000 + i;
                 uint64_t t;
-                if (core.submit(r, (1 + (i % 3)) * MB, 5, 5, now_s(), false, nullptr, &t) != S_OK) continue;
+                if (core.submit(r, (1 + (i % 3)) * MB, 5, 5, 0, now_s(), false, nullptr, &t) != S_OK) continue;
                 if (core.wait(t, 10.0) == S_OK && core.release(t) == S_OK) ++ok;
             }
         });
@@ -327,6 +354,7 @@ int main() {
     CHECK(class_bytes(class_of(16588800)) < 16588800ull * 6 / 5);   // 1080p colour+depth: < 20% slack
     test_fifo_and_batching();
     test_oversized_single_request_runs_alone();
+    test_pixel_cap();
     test_budget_defers_and_rejects();
     test_eviction_of_idle_blocks();
     test_failures();

@@ -149,3 +149,17 @@ def test_orbit_camera_looks_at_target():
     view, proj, vp, _ = H.R.camera_matrices(pos, (0, 0, 0), 60.0, 16 / 9)
     c = vp.reshape(4, 4).T.astype(np.float64) @ np.array([0, 0, 0, 1.0])
     assert abs(c[0] / c[3]) < 1e-5 and abs(c[1] / c[3]) < 1e-5   # target at the screen centre
+
+
+def test_view_limits():
+    """A view submits fewer than 2^29 faces (the 32-bit tie-break key is
+    rank * 8 + clip sub-triangle) on a framebuffer of 1..32768 pixels per
+    edge; axr_draw, batch creation and server submission apply this check."""
+    lib = N.lib()
+    assert lib.axr_check_view(0, 1, 1) == N.AXR_OK
+    assert lib.axr_check_view((1 << 29) - 1, 32768, 32768) == N.AXR_OK
+    # the largest key a view can produce still fits 32 bits
+    assert ((1 << 29) - 1) * 8 + 7 == 0xFFFFFFFF
+    for faces, w, h in (((1 << 29), 64, 64), ((1 << 32), 64, 64), ((1 << 63), 64, 64), (10, 0, 64), (10, 64, 0),
+                        (10, 32769, 64), (10, 64, 32769), (10, -1, 64)):
+        assert lib.axr_check_view(faces, w, h) == N.AXR_ERR_INVALID, (faces, w, h)

@@ -275,7 +275,21 @@ def test_async_frames_without_ids_shade_ahead(width, height, tile):
         ctx.close()
 
 
-def _frames_without_ids(ctx, sc):
+@pytest.mark.parametrize("width,height,tile", [(200, 150, 16), (333, 197, 8), (250, 130, 32)])
+def test_async_frames_with_ids_two_pass(width, height, tile):
+    """The same frames into framebuffers with an id plane: the two-pass
+    resolve keeps each winner's key in its slot and its colour in the
+    arena's colour slots, and the finishing pass writes all three planes
+    (16-byte writes when the width is a multiple of 4, else 4-byte)."""
+    ctx = R.Context(0)
+    try:
+        ctx.set_config(tile_size=tile, raster_mode=N.RASTER_VISIBILITY)
+        _frames_without_ids(ctx, S.icosphere_scene(n=16, width=width, height=height, tex=64), with_ids=True)
+    finally:
+        ctx.close()
+
+
+def _frames_without_ids(ctx, sc, with_ids=False):
     d = median_depth(sc)
     light = (0.2, -1.0, -0.3)
     variants = {4: sc,
@@ -283,7 +297,7 @@ def _frames_without_ids(ctx, sc):
                 1: S.with_shader(sc, {"kind": 1, "lightDirection": light, "lightColor": (0.6, 0.6, 0.6)}),
                 3: S.with_shader(sc, {"kind": 3, "lightDirection": light})}
     gs = H.GpuScene(ctx, sc)
-    fbs = [R.Framebuffer(ctx, sc.width, sc.height) for _ in range(3)]
+    fbs = [R.Framebuffer(ctx, sc.width, sc.height, with_ids=with_ids) for _ in range(3)]
     steps = [[] for _ in fbs]
     try:
         for i, kind in enumerate([4, 0, 1, 3, 4, 4, 1, 4, 0]):
@@ -304,7 +318,9 @@ def _frames_without_ids(ctx, sc):
         gs.close()
     for k in range(3):
         ref = oracle_sequence(sc, steps[k])
-        H.assert_parity(H.compare(got[k], (ref[0], ref[1], None)), f"framebuffer {k}")
+        H.assert_parity(H.compare(got[k], ref if with_ids else (ref[0], ref[1], None)), f"framebuffer {k}")
+        if with_ids:
+            assert (got[k][2] != H.EMPTY_ID).sum() > 100
 
 
 @pytest.mark.parametrize("tile", [8, 16])
