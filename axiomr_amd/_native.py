@@ -147,6 +147,7 @@ SIGNATURES = {
     "axr_fb_read": (C.c_int, [_P, _U8, _F, _U32]),
     "axr_draw": (C.c_int, [_P, _P, _P, C.POINTER(AxrDrawParams)]),
     "axr_check_view": (C.c_int, [C.c_uint64, C.c_int, C.c_int]),
+    "axr_draw_noalloc": (C.c_int, [_P, _P, _P, C.POINTER(AxrDrawParams)]),
     "axr_batch_create": (C.c_int, [_P, C.c_uint32, _PP, _PP, C.POINTER(AxrDrawParams), _PP]),
     "axr_batch_set_clear": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_float]),
     "axr_batch_launch": (C.c_int, [_P]),

@@ -32,7 +32,8 @@ DEVICE = [
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["raster.hip", "capi.hip", "serve.hip", "scene.cpp", "camera.cpp", "ingest.cpp", "image.cpp", "jpeg.cpp"]
+SOURCES = ["raster.hip", "capi.hip", "serve.hip", "noalloc.hip", "scene.cpp", "camera.cpp", "ingest.cpp", "image.cpp",
+           "jpeg.cpp"]
 
 
 def _compile(src: str) -> Path:

@@ -531,6 +531,10 @@ int axr_ctx_create(int device, uint64_t hbm_budget_bytes, axr_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return AXR_ERR_HIP; }
     for (auto& e : ctx->join_ev)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { axr_ctx_destroy(ctx); return AXR_ERR_HIP; }
+    if (hipHostMalloc((void**)&ctx->noalloc_counters, CNT_COUNT * 4, hipHostMallocDefault) != hipSuccess) {
+        axr_ctx_destroy(ctx);
+        return AXR_ERR_OOM;
+    }
     int r = arena_create(ctx, ctx->err, ctx->main);
     if (r != AXR_OK) { axr_ctx_destroy(ctx); return r; }
     *out = ctx;
@@ -547,6 +551,7 @@ int axr_ctx_destroy(axr_ctx* ctx) {
     arena_destroy(ctx, ctx->main);
     for (auto& e : ctx->join_ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->noalloc_counters) (void)hipHostFree(ctx->noalloc_counters);
     if (ctx->ev) {
         for (int i = 0; i < axr_ctx::kEventRing; ++i)
             for (int k = 0; k < 5; ++k) {
@@ -642,7 +647,8 @@ int axr_sync(axr_ctx* ctx) {
         const int r2 = a.stream ? arena_settle_overflow(ctx, a) : AXR_OK;
         if (!r) r = r2;
     }
-    return r;
+    const int r3 = noalloc_settle(ctx);
+    return r ? r : r3;
 }
 
 // ---- resources -------------------------------------------------------------

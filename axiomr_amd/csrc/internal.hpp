@@ -105,6 +105,10 @@ struct axr_ctx {
     axr::StageEvents* ev = nullptr;     // kEventRing entries, created on first use
     int ev_head = 0, ev_pending = 0;
     axr_kernel_stats stats{};
+    // axr_draw_noalloc: a call since the last sync may have left an overflow
+    // in the device's scratch counters; pinned copy of those counters
+    bool noalloc_pending = false;
+    uint32_t* noalloc_counters = nullptr;
 };
 
 struct axr_texture {
@@ -185,6 +189,9 @@ int ctx_join(axr_ctx* ctx);
 int ctx_sync_all(axr_ctx* ctx);
 // Write a framebuffer's pending clear, if any (stream-ordered).
 int fb_settle(axr_fb* fb);
+// After a host wait: the overflow an asynchronous axr_draw_noalloc left in
+// the device's scratch counters (AXR_OK or AXR_ERR_OVERFLOW).
+int noalloc_settle(axr_ctx* ctx);
 // The shader kind all n views use, or -1 if they differ.
 int uniform_shader(const ViewDesc* views, uint32_t n);
 ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,

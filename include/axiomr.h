@@ -183,6 +183,22 @@ int axr_fb_read(axr_fb* fb, uint8_t* bgra, float* depth, uint32_t* ids);
 /* reference TiledPipeline::drawMesh (src/tiled_pipeline.cpp:143-322). */
 int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_params* params);
 
+/* axr_draw without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory while
+ * they draw (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same visibility-mode kernels and pixels as
+ * axr_draw, on the context's own stream, with the frame's working memory
+ * carved out of one fixed 64 MiB device array per GPU (held by the process
+ * from library load until it exits) instead of the context's frame arenas.
+ * A frame that needs more than 64 MiB (about 1920x1080 with 1M faces needs
+ * 30 MB; the per-pixel keys take 8 bytes a pixel) fails with
+ * AXR_ERR_INVALID before any work is queued; a frame whose clipped or
+ * large triangles overflow their share is not drawn and reports
+ * AXR_ERR_OVERFLOW (at the call, or at axr_sync with AXR_DRAW_ASYNC).  All
+ * calls on one GPU share its array: calls on one context are ordered by its
+ * stream, calls from different contexts or threads that overlap race. */
+int axr_draw_noalloc(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_params* params);
+
 /* Size limits of one view (one drawMesh), checked by axr_draw, batch
  * creation and server submission: AXR_OK, or AXR_ERR_INVALID when the view
  * submits 2^29 faces or more (the depth tie-break key, submission rank * 8
