@@ -60,9 +60,10 @@ class AxrConfig(C.Structure):
 
 
 class AxrKernelStats(C.Structure):
-    _fields_ = [("frames", C.c_uint64), ("frame_ms_total", C.c_double), ("stage_ms_total", C.c_double * 5)]
+    _fields_ = [("frames", C.c_uint64), ("frame_ms_total", C.c_double), ("stage_ms_total", C.c_double * 6),
+                ("passes_skipped", C.c_uint64), ("skip_faults", C.c_uint64)]
 
-    STAGES = ("setup", "clip", "scan", "scatter", "raster")
+    STAGES = ("setup", "clip", "scan", "scatter", "raster", "vertex pass")
 
 
 class AxrServerConfig(C.Structure):

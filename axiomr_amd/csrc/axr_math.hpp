@@ -29,6 +29,29 @@
 
 namespace axr {
 
+// p[i], loaded through the global address space.  The kernels read many
+// pointers from memory (a batch's view table, a material, a texture
+// descriptor); to the compiler those are generic pointers, and a load
+// through one is a flat load, which waits on the LDS and scalar counters as
+// well as the vector-memory one.  They always point to global memory.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+__device__ __forceinline__ T gld(const T* p, size_t i) {
+    T v;
+    __builtin_memcpy(&v, (const __attribute__((address_space(1))) T*)p + i, sizeof(T));
+    return v;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, size_t i, const T& v) {
+    __builtin_memcpy((__attribute__((address_space(1))) T*)p + i, &v, sizeof(T));
+}
+#else
+template <class T>
+AXR_HD T gld(const T* p, size_t i) { return p[i]; }
+template <class T>
+AXR_HD void gst(T* p, size_t i, const T& v) { p[i] = v; }
+#endif
+
 struct vec2 { float x, y; };
 struct vec3 { float x, y, z; };
 struct vec4 { float x, y, z, w; };

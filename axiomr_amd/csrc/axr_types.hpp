@@ -79,6 +79,9 @@ struct ViewDesc {
     const uint32_t* face_colors;   // RGBA8 per face or null
     const DevGroup* groups;
     const DevMaterial* materials;
+    const void* vary;              // the vertex stage's output for every vertex of the
+                                   // mesh (the shader's Var type), computed by k_vary
+                                   // earlier in the same launch; null: run it per pixel
     DevGroup g0;                   // groups[0], for single-group meshes without a load
     uint32_t ngroups;
     uint32_t nfaces;               // faces in submission order (sum of group counts)
@@ -145,6 +148,7 @@ constexpr uint32_t ERR_CLIP_FACES = 1u;
 constexpr uint32_t ERR_CLIP_RECS = 2u;
 constexpr uint32_t ERR_PAIRS = 4u;
 constexpr uint32_t ERR_BIG = 8u;
+constexpr uint32_t ERR_SKIPPED = 16u;   // a pass the host skipped had work: the frame is redrawn with it
 
 // Raster modes (axr_config.raster_mode).
 enum RasterMode : int32_t {
@@ -192,6 +196,18 @@ struct Arena {
     uint32_t pad;
 };
 
+// One run of the vertex stage over a mesh's vertices for the views that
+// share it (batched form): vertices [first, first + count) of the launch's
+// numbering, the uniforms of view `view`, the shader's Var per vertex at out.
+struct VaryJob {
+    const Vertex56* verts;
+    void* out;
+    uint32_t first;
+    uint32_t count;
+    uint32_t view;
+    int32_t kind;
+};
+
 // Kernel arguments common to every stage.  `table` points to `nviews`
 // descriptors in device memory; when it is null the single descriptor `one`
 // (passed by value) is the only view.
@@ -217,6 +233,14 @@ struct KArgs {
     int32_t two_pass = 0;       // plain visibility-mode draw with a clear: shade
                                 // (k_shade) apart from the framebuffer writes
                                 // (k_finish), which alone wait for earlier draws
+    // batches: the vertex stage once per vertex of each shared mesh (k_vary)
+    const VaryJob* vary_jobs = nullptr;
+    const uint32_t* vary_firsts = nullptr;
+    uint32_t vary_njobs = 0;
+    uint32_t vary_total = 0;
+    int32_t skip_clip = 0;      // one view: no face can cross a frustum plane, so the
+    int32_t skip_big = 0;       // clipper (and, when no triangle can exceed the
+                                // footprint, the large-triangle pass) is not launched
     int32_t depth_is_clear = 0; // binned mode: every framebuffer's depth plane holds
                                 // clear_depth (written just before, by k_clear /
                                 // k_clear_views): winners are tested against that

@@ -24,6 +24,12 @@ struct axr_batch {
     uint32_t* d_pix_bases = nullptr;
     uint32_t* d_tile_views = nullptr;
     uint32_t total_pixels = 0;
+    // the vertex stage once per vertex of each mesh the views share (k_vary):
+    // one job per (mesh, shader kind, the uniforms the vertex stage reads)
+    std::vector<VaryJob> vary_jobs;
+    std::vector<size_t> vary_bytes;     // the output buffer of each job
+    VaryJob* d_vary_jobs = nullptr;
+    uint32_t* d_vary_firsts = nullptr;
     bool clear = false;            // clear every view's framebuffer before drawing
     uint32_t clear_bgra = 0;
     float clear_depth = 0.0f;
@@ -335,6 +341,11 @@ int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, StageEvents* ev,
 int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs) {
     const uint32_t* c = fa.host_counters;
     if (!c[CNT_ERROR]) return AXR_OK;
+    // a pass the host judged empty had work: never skip passes again here
+    if (c[CNT_ERROR] & ERR_SKIPPED) {
+        ctx->no_pass_skips = true;
+        ++ctx->stats.skip_faults;
+    }
     auto grown = [](uint64_t n, uint64_t num, uint64_t den) { return n ? n + n * num / den : 0; };
     ArenaNeeds need;
     need.clip_faces = grown(c[CNT_NEED_CLIP_FACES], 1, 2);
@@ -408,6 +419,48 @@ int fb_settle(axr_fb* fb) {
     return AXR_OK;
 }
 
+PassSkips pass_skips(const axr_mesh* m, const mat4& mvp, int W, int H, int footprint) {
+    PassSkips s;
+    if (!std::isfinite(m->radius)) return s;
+    // row k of the clip transform: clip_k = r[k] . (p, 1)
+    double r[4][4];
+    for (int k = 0; k < 4; ++k) {
+        const float* e = &mvp.c[0].x;
+        for (int c = 0; c < 4; ++c) r[k][c] = e[4 * c + k];
+    }
+    auto at_center = [&](const double* q) { return q[0] * m->center[0] + q[1] * m->center[1] + q[2] * m->center[2] + q[3]; };
+    auto slope = [&](const double* q) { return std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]); };
+    // A linear function of p over the sphere lies within value(centre) +- slope * radius.
+    // Float rounding of the kernels' transform and plane distances is far
+    // below the margin (1e-4 of the magnitudes involved).
+    auto margin = [&](const double* q) { return 1e-4 * (slope(q) * (std::fabs(m->center[0]) + std::fabs(m->center[1]) +
+                                                                     std::fabs(m->center[2]) + m->radius) + std::fabs(q[3])); };
+    for (int plane = 0; plane < 6; ++plane) {
+        // x + w, w - x, y + w, w - y, z + w, w - z (reference pipeline.cpp:274-285)
+        const int axis = plane / 2;
+        const double sg = (plane % 2) ? -1.0 : 1.0;
+        double q[4];
+        for (int c = 0; c < 4; ++c) q[c] = r[3][c] + sg * r[axis][c];
+        if (!(at_center(q) - slope(q) * m->radius > margin(q))) return s;
+    }
+    s.clip = true;
+    // Screen extent of any face: for p, q within max_edge of each other in
+    // the sphere, |x_ndc(p) - x_ndc(q)| <= max_edge * sup |grad(a.P / b.P)|
+    // over the segment, with grad = (a' (b.P) - b' (a.P)) / (b.P)^2.  The
+    // segment lies inside the frustum (every point of the sphere is), where
+    // |a.P| <= b.P = w, so |grad| <= (|a'| + |b'|) / w <= (|a'| + |b'|) / w_lo.
+    const double w_lo = at_center(r[3]) - slope(r[3]) * m->radius;
+    if (!(w_lo > 0)) return s;
+    auto extent = [&](const double* a, int size) {
+        const double g = (slope(a) + slope(r[3])) / w_lo;
+        return 0.5 * size * m->max_edge * g * (1.0 + 1e-4) + 1e-3;
+    };
+    // the pixel rectangle spans floor(min)..ceil(max): at most extent + 2 pixels
+    const double ex = std::ceil(extent(r[0], W)) + 2.0, ey = std::ceil(extent(r[1], H)) + 2.0;
+    s.big = ex * ey <= (double)footprint;
+    return s;
+}
+
 int uniform_shader(const ViewDesc* views, uint32_t n) {
     for (uint32_t i = 1; i < n; ++i)
         if (views[i].u.kind != views[0].u.kind) return -1;
@@ -472,7 +525,7 @@ void harvest_events(axr_ctx* ctx) {
         (void)hipEventElapsedTime(&total, e.start[first], e.stop[last]);
         ctx->stats.frames += 1;
         ctx->stats.frame_ms_total += total;
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < kStages; ++k) {
             if (!((e.used >> k) & 1u)) continue;
             float ms = 0;
             (void)hipEventElapsedTime(&ms, e.start[k], e.stop[k]);
@@ -554,7 +607,7 @@ int axr_ctx_destroy(axr_ctx* ctx) {
     if (ctx->noalloc_counters) (void)hipHostFree(ctx->noalloc_counters);
     if (ctx->ev) {
         for (int i = 0; i < axr_ctx::kEventRing; ++i)
-            for (int k = 0; k < 5; ++k) {
+            for (int k = 0; k < kStages; ++k) {
                 if (ctx->ev[i].start[k]) (void)hipEventDestroy(ctx->ev[i].start[k]);
                 if (ctx->ev[i].stop[k]) (void)hipEventDestroy(ctx->ev[i].stop[k]);
             }
@@ -596,7 +649,7 @@ int axr_ctx_set_profiling(axr_ctx* ctx, int enable) {
         ctx->ev = new (std::nothrow) StageEvents[axr_ctx::kEventRing];
         if (!ctx->ev) return fail(ctx->err, AXR_ERR_OOM, "profiling events");
         for (int i = 0; i < axr_ctx::kEventRing; ++i)
-            for (int k = 0; k < 5; ++k)
+            for (int k = 0; k < kStages; ++k)
                 if (hipEventCreate(&ctx->ev[i].start[k]) != hipSuccess ||
                     hipEventCreate(&ctx->ev[i].stop[k]) != hipSuccess)
                     return fail(ctx->err, AXR_ERR_HIP, "hipEventCreate");
@@ -746,6 +799,31 @@ int axr_mesh_create(axr_ctx* ctx, const float* verts56, uint64_t nverts, const u
         return r;
     }
     for (size_t s : sizes) m->bytes += s;
+    // bounds for pass_skips: centre of the axis-aligned box, radius to the
+    // farthest vertex, longest edge
+    double lo[3] = {verts56[0], verts56[1], verts56[2]}, hi[3] = {lo[0], lo[1], lo[2]};
+    for (uint64_t i = 0; i < nverts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], (double)verts56[14 * i + k]);
+            hi[k] = std::max(hi[k], (double)verts56[14 * i + k]);
+        }
+    for (int k = 0; k < 3; ++k) m->center[k] = 0.5 * (lo[k] + hi[k]);
+    double r2 = 0, e2 = 0;
+    for (uint64_t i = 0; i < nverts; ++i) {
+        double d = 0;
+        for (int k = 0; k < 3; ++k) d += (verts56[14 * i + k] - m->center[k]) * (verts56[14 * i + k] - m->center[k]);
+        r2 = std::max(r2, d);
+    }
+    for (uint64_t f = 0; f < nfaces; ++f)
+        for (int k = 0; k < 3; ++k) {
+            const float* a = verts56 + 14 * (uint64_t)idx[3 * f + k];
+            const float* b = verts56 + 14 * (uint64_t)idx[3 * f + (k + 1) % 3];
+            const double dx = (double)a[0] - b[0], dy = (double)a[1] - b[1], dz = (double)a[2] - b[2];
+            e2 = std::max(e2, dx * dx + dy * dy + dz * dz);
+        }
+    m->radius = std::sqrt(r2);
+    m->max_edge = std::sqrt(e2);
+    if (!std::isfinite(m->radius) || !std::isfinite(m->max_edge)) m->radius = m->max_edge = INFINITY;
     *out = m;
     return AXR_OK;
 }
@@ -939,6 +1017,12 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
     ka.mode = ctx->cfg.raster_mode;
     ka.total_pixels = (uint32_t)((uint64_t)fb->w * (uint64_t)fb->h);
     ka.shader = p->shader;
+    if (!ctx->no_pass_skips) {
+        const PassSkips sk = pass_skips(mesh, ka.one.mvp, fb->w, fb->h, ka.small_footprint);
+        ka.skip_clip = sk.clip;
+        ka.skip_big = sk.big && ka.mode == RASTER_VISIBILITY;
+        ctx->stats.passes_skipped += (uint64_t)ka.skip_clip + (uint64_t)ka.skip_big;
+    }
     if (fb->clear_pending && ka.mode == RASTER_VISIBILITY) {
         // the resolve pass writes the clear (k_resolve, ka.clear)
         ka.clear = 1;
@@ -977,6 +1061,46 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
 }
 
 // ---- batched form --------------------------------------------------------------
+
+// The uniforms a vertex stage reads are equal: the model matrix (normal-mapped
+// and Gouraud shaders), the normal matrix (flat shader), the light direction
+// (Gouraud).
+static bool same_vertex_uniforms(const Uniforms& a, const Uniforms& b) {
+    return std::memcmp(&a.model, &b.model, sizeof(a.model)) == 0 &&
+           std::memcmp(&a.normal_mat, &b.normal_mat, sizeof(a.normal_mat)) == 0 &&
+           std::memcmp(&a.light_dir, &b.light_dir, sizeof(a.light_dir)) == 0;
+}
+
+// Groups the batch's views into vertex-stage jobs (views of one mesh with
+// one shader kind and equal vertex-stage uniforms share one), allocates each
+// job's output and points the views at it.  The batch's launch runs k_vary
+// over all jobs before the resolve reads them.
+static int batch_plan_vary(axr_batch* b, const axr_mesh* const* meshes) {
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < b->nviews; ++i) {
+        ViewDesc& v = b->views[i];
+        const uint32_t vb = vary_bytes(v.u.kind);
+        if (!vb) continue;
+        size_t j = 0;
+        for (; j < b->vary_jobs.size(); ++j) {
+            const VaryJob& q = b->vary_jobs[j];
+            if (q.verts == v.verts && q.kind == v.u.kind && same_vertex_uniforms(b->views[q.view].u, v.u)) break;
+        }
+        if (j == b->vary_jobs.size()) {
+            const uint64_t nv = meshes[i]->nverts;
+            if (total + nv > 0x7FFFFFFFull) continue;   // the rest run their vertex stage per pixel
+            void* out = nullptr;
+            const size_t bytes = (size_t)nv * vb;
+            int r = dev_alloc(b->ctx, b->ctx->err, &out, bytes);
+            if (r) return r;
+            b->vary_jobs.push_back(VaryJob{v.verts, out, (uint32_t)total, (uint32_t)nv, i, v.u.kind});
+            b->vary_bytes.push_back(bytes);
+            total += nv;
+        }
+        v.vary = b->vary_jobs[j].out;
+    }
+    return AXR_OK;
+}
 
 int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const axr_mesh* const* meshes,
                      const axr_draw_params* params, axr_batch** out) {
@@ -1035,6 +1159,11 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     tbase[nviews] = (uint32_t)tiles;
     pbase[nviews] = (uint32_t)pixels;
     b->total_pixels = (uint32_t)pixels;
+    int r;
+    if ((r = batch_plan_vary(b, meshes))) {
+        axr_batch_destroy(b);
+        return r;
+    }
     std::vector<uint32_t> tview(tiles);
     for (uint32_t i = 0; i < nviews; ++i)
         for (uint32_t t = tbase[i]; t < tbase[i + 1]; ++t) tview[t] = i;
@@ -1043,7 +1172,6 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     ErrSink& err = ctx->err;
     KArgs& ka = b->ka;
     ka.total_tiles = (uint32_t)tiles;
-    int r;
     if ((r = dev_alloc(ctx, err, (void**)&b->d_views, sz_v)) ||
         (r = dev_alloc(ctx, err, (void**)&b->d_face_bases, sz_b)) ||
         (r = dev_alloc(ctx, err, (void**)&b->d_tile_bases, sz_b)) ||
@@ -1058,6 +1186,22 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
         return r;
     }
     b->bytes = sz_v + 3 * sz_b + tiles * 4;
+    if (!b->vary_jobs.empty()) {
+        const uint32_t nj = (uint32_t)b->vary_jobs.size();
+        std::vector<uint32_t> firsts(nj);
+        for (uint32_t j = 0; j < nj; ++j) firsts[j] = b->vary_jobs[j].first;
+        if ((r = dev_alloc(ctx, err, (void**)&b->d_vary_jobs, nj * sizeof(VaryJob))) ||
+            (r = dev_alloc(ctx, err, (void**)&b->d_vary_firsts, nj * 4)) ||
+            (r = upload(err, st, b->d_vary_jobs, b->vary_jobs.data(), nj * sizeof(VaryJob))) ||
+            (r = upload(err, st, b->d_vary_firsts, firsts.data(), nj * 4))) {
+            axr_batch_destroy(b);
+            return r;
+        }
+        ka.vary_jobs = b->d_vary_jobs;
+        ka.vary_firsts = b->d_vary_firsts;
+        ka.vary_njobs = nj;
+        ka.vary_total = b->vary_jobs.back().first + b->vary_jobs.back().count;
+    }
     ka.table = b->d_views;
     ka.face_bases = b->d_face_bases;
     ka.tile_bases = b->d_tile_bases;
@@ -1106,13 +1250,16 @@ int axr_batch_launch(axr_batch* b) {
         ka.clear_bgra = b->clear_bgra;
         ka.clear_depth = b->clear_depth;
     }
-    if (b->clear && !absorb) {
-        // k_clear_views writes the clear just before the frame's kernels
+    // k_clear_views writes the clear just before the frame's kernels (writing
+    // it from the setup grid instead, beside the face work, took the setup
+    // from 0.50 to 0.91 ms on the 256-view batch: slower than the 0.2 ms pass)
+    const bool clear_pass = b->clear && !absorb;
+    if (clear_pass) {
         ka.depth_is_clear = 1;
         ka.clear_depth = b->clear_depth;
     }
     if (ctx->profiling) {   // events need the eager path
-        if (b->clear && !absorb) {
+        if (clear_pass) {
             hipError_t e = launch_clear_views(b->d_views, b->d_pix_bases, b->nviews, b->total_pixels, b->clear_bgra,
                                               b->clear_depth, fa.stream);
             if (e != hipSuccess) return hip_fail(ctx->err, e, "batch clear");
@@ -1130,7 +1277,7 @@ int axr_batch_launch(axr_batch* b) {
         hipError_t e = hipStreamBeginCapture(fa.stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(ctx->err, e, "hipStreamBeginCapture");
         hipError_t le = hipSuccess;
-        if (b->clear && !absorb)
+        if (clear_pass)
             le = launch_clear_views(b->d_views, b->d_pix_bases, b->nviews, b->total_pixels, b->clear_bgra,
                                     b->clear_depth, fa.stream);
         if (le == hipSuccess) le = launch_frame(ka, fa.a, fa.stream, nullptr);
@@ -1168,6 +1315,9 @@ int axr_batch_destroy(axr_batch* b) {
     dev_free(ctx, b->d_tile_bases, sz_b);
     dev_free(ctx, b->d_pix_bases, sz_b);
     dev_free(ctx, b->d_tile_views, (size_t)b->ka.total_tiles * 4);
+    for (size_t j = 0; j < b->vary_jobs.size(); ++j) dev_free(ctx, b->vary_jobs[j].out, b->vary_bytes[j]);
+    dev_free(ctx, b->d_vary_jobs, b->vary_jobs.size() * sizeof(VaryJob));
+    dev_free(ctx, b->d_vary_firsts, b->vary_jobs.size() * 4);
     if (b->holds_refs) {
         for (axr_fb* fb : b->fbs) --fb->batch_refs;
         for (const axr_mesh* m : b->meshes) --const_cast<axr_mesh*>(m)->batch_refs;

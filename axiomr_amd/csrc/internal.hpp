@@ -26,9 +26,10 @@ struct FbWait {
 // (hipExtLaunchKernel), so stop - start is the stage's kernel time as the
 // command processor sees it (the figure a kernel trace reports), without
 // the launch gap before it.  `used` has bit k set for every stage launched.
+constexpr int kStages = 6;
 struct StageEvents {
-    hipEvent_t start[5] = {};
-    hipEvent_t stop[5] = {};
+    hipEvent_t start[kStages] = {};
+    hipEvent_t stop[kStages] = {};
     uint32_t used = 0;
 };
 
@@ -40,6 +41,8 @@ hipError_t launch_clear(uint32_t* color, float* depth, uint32_t* ids, size_t n, 
 hipError_t launch_prep(const Arena& ar, uint32_t ntiles, hipStream_t st);
 hipError_t launch_clear_views(const ViewDesc* views, const uint32_t* pix_bases, uint32_t nviews, uint32_t total,
                               uint32_t bgra, float d, hipStream_t st);
+// Bytes of a shader kind's vertex-stage output per vertex (0: none).
+uint32_t vary_bytes(int kind);
 
 struct DevBuf {
     void* p = nullptr;
@@ -99,6 +102,7 @@ struct axr_ctx {
     uint64_t last_value = 0;            // nonzero: that work was a draw whose stream then wrote this
                                         // value to its arena's signal (0: wait with an event)
     bool profiling = false;
+    bool no_pass_skips = false;         // set if a pass judged empty on the host ever had work
     // timed frames not yet read back; the ring holds the frames of a whole
     // timed run, so timing does not stall the host until stats are read
     static constexpr int kEventRing = 1024;
@@ -131,6 +135,11 @@ struct axr_mesh {
     uint32_t batch_refs = 0;   // views of live batches that draw this mesh
     axr::DevGroup g0{};        // first group (host copy)
     size_t bytes = 0;
+    // model-space bounds, for deciding on the host that a draw needs no
+    // clipper or large-triangle pass: a sphere holding every vertex and the
+    // longest edge of any face
+    double center[3] = {0, 0, 0};
+    double radius = 0, max_edge = 0;
 };
 
 struct axr_fb {
@@ -194,6 +203,13 @@ int fb_settle(axr_fb* fb);
 int noalloc_settle(axr_ctx* ctx);
 // The shader kind all n views use, or -1 if they differ.
 int uniform_shader(const ViewDesc* views, uint32_t n);
+// Which of the passes after setup a draw of `m` with `mvp` on a W x H
+// target can be shown to have no work (conservative: false when unsure).
+struct PassSkips {
+    bool clip = false;   // no face crosses a frustum plane
+    bool big = false;    // and no triangle's pixel rectangle exceeds `footprint`
+};
+PassSkips pass_skips(const axr_mesh* m, const mat4& mvp, int W, int H, int footprint);
 ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,
                    const axr_draw_params* p, int tile);
 bool shader_ok(const axr_draw_params* p);

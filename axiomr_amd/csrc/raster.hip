@@ -85,13 +85,13 @@ __device__ __forceinline__ DevGroup group_of(const ViewDesc& V, uint32_t rank) {
     uint32_t lo = 0, hi = V.ngroups;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
-        if (V.groups[mid].rank_base <= rank) lo = mid; else hi = mid;
+        if (gld(V.groups, mid).rank_base <= rank) lo = mid; else hi = mid;
     }
-    return V.groups[lo];
+    return gld(V.groups, lo);
 }
 
 __device__ __forceinline__ vec3 load_pos(const float* p, uint32_t i) {
-    return make3(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]);
+    return gld(reinterpret_cast<const vec3*>(p), i);
 }
 
 // One face as the setup stage sees it: submission rank s within its view,
@@ -106,10 +106,10 @@ struct FaceGeo {
 __device__ __forceinline__ void face_geo(const ViewDesc& V, uint32_t s, FaceGeo& f) {
     f.G = group_of(V, s);
     f.face = f.G.first + (s - f.G.rank_base);
-    const uint32_t* ip = V.idx + 3 * (size_t)f.face;
-    f.i0 = ip[0];
-    f.i1 = ip[1];
-    f.i2 = ip[2];
+    const uint3 ix = gld(reinterpret_cast<const uint3*>(V.idx), f.face);
+    f.i0 = ix.x;
+    f.i1 = ix.y;
+    f.i2 = ix.z;
     f.c0 = V.mvp * make4(load_pos(V.pos, f.i0), 1.0f);
     f.c1 = V.mvp * make4(load_pos(V.pos, f.i1), 1.0f);
     f.c2 = V.mvp * make4(load_pos(V.pos, f.i2), 1.0f);
@@ -492,6 +492,7 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
     if (want_clip) {
         if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g;
         else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
+        if (ka.skip_clip) atomicOr(&ar.counters[CNT_ERROR], ERR_SKIPPED);
     }
     emit_pairs_wave<T>(ar, view_at<BATCH>(ka, vi), binnable, rc, g, g >> 6);
 }
@@ -527,10 +528,10 @@ __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
         if (act[k]) {
             G[k] = group_of(V, s[k]);
             face[k] = G[k].first + (s[k] - G[k].rank_base);
-            const uint32_t* ip = V.idx + 3 * (size_t)face[k];
-            i0[k] = ip[0];
-            i1[k] = ip[1];
-            i2[k] = ip[2];
+            const uint3 ix = gld(reinterpret_cast<const uint3*>(V.idx), face[k]);
+            i0[k] = ix.x;
+            i1[k] = ix.y;
+            i2[k] = ix.z;
         }
     }
     vec3 p0[F], p1[F], p2[F];
@@ -579,6 +580,7 @@ __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
             if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g[k];
             else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
         }
+        if ((want_clip && ka.skip_clip) || (big && ka.skip_big)) atomicOr(&ar.counters[CNT_ERROR], ERR_SKIPPED);
         big_append_wave(ar, big, r, area);
     }
 }
@@ -685,9 +687,9 @@ __global__ __launch_bounds__(64) void k_clip(KArgs ka, Arena ar) {
         const DevGroup G = group_of(V, s);
         const uint32_t face = G.first + (s - G.rank_base);
         for (int k = 0; k < 3; ++k) {
-            const uint32_t vtx = V.idx[3 * (size_t)face + k];
-            const float* src = reinterpret_cast<const float*>(V.verts + vtx);
-            for (int i = 0; i < 14; ++i) tris[k].f[i] = src[i];
+            const uint32_t vtx = gld(V.idx, 3 * (size_t)face + k);
+            const Vertex56 src = gld(V.verts, vtx);
+            __builtin_memcpy(tris[k].f, &src, sizeof(Vertex56));
             vec4 c = V.mvp * make4(load_pos(V.pos, vtx), 1.0f);
             tris[k].f[14] = c.x; tris[k].f[15] = c.y; tris[k].f[16] = c.z; tris[k].f[17] = c.w;
         }
@@ -917,17 +919,23 @@ __device__ __forceinline__ vec4 shade_at(const ViewDesc& V, const Arena& ar, uin
                                          vec3 bar) {
     ShadeCtx c;
     c.u = &V.u;
-    c.m = &V.materials[material & REC_MATERIAL_MASK];
+    const DevMaterial mat = gld(V.materials, material & REC_MATERIAL_MASK);
+    c.m = &mat;
     c.face_colors = V.face_colors;
     c.prim = prim;
     typename S::Var var[3];
     if constexpr (sizeof(typename S::Var) > 1) {
         if (clipped) {
             for (int k = 0; k < 3; ++k) var[k] = S::vs(ar.clip_verts[3 * (size_t)slot + k], V.u);
+        } else if (V.vary) {
+            const typename S::Var* vy = static_cast<const typename S::Var*>(V.vary);
+            var[0] = gld(vy, i0);
+            var[1] = gld(vy, i1);
+            var[2] = gld(vy, i2);
         } else {
-            var[0] = S::vs(V.verts[i0], V.u);
-            var[1] = S::vs(V.verts[i1], V.u);
-            var[2] = S::vs(V.verts[i2], V.u);
+            var[0] = S::vs(gld(V.verts, i0), V.u);
+            var[1] = S::vs(gld(V.verts, i1), V.u);
+            var[2] = S::vs(gld(V.verts, i2), V.u);
         }
     }
     return S::fs(bar, var, c);
@@ -943,10 +951,10 @@ __device__ __forceinline__ vec4 shade_winner(const ViewDesc& V, const Arena& ar,
     } else {
         prim = r.c.z;
         if constexpr (sizeof(typename S::Var) > 1) {
-            const uint32_t* ip = V.idx + 3 * (size_t)prim;
-            i0 = ip[0];
-            i1 = ip[1];
-            i2 = ip[2];
+            const uint3 ix = gld(reinterpret_cast<const uint3*>(V.idx), prim);
+            i0 = ix.x;
+            i1 = ix.y;
+            i2 = ix.z;
         }
     }
     return shade_at<S>(V, ar, r.c.w, prim, clipped, slot, i0, i1, i2, bar);
@@ -972,11 +980,11 @@ __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar
     float al, be, ga, z;
     cover(rt, px, py, al, be, ga, z);
     const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
-    if (!(z < (known ? known_depth : V.depth[pix]))) return;   // reference mergeTileResults: strict less
+    if (!(z < (known ? known_depth : gld(V.depth, pix)))) return;   // reference mergeTileResults: strict less
     const vec4 col = shade_winner<S>(V, ar, r, slot, clipped, rank, make3(al, be, ga));
-    V.depth[pix] = z;
-    V.color[pix] = pack_bgra(col);
-    if (V.ids) V.ids[pix] = key;
+    gst(V.depth, pix, z);
+    gst(V.color, pix, pack_bgra(col));
+    if (V.ids) gst(V.ids, pix, key);
 }
 
 template <int T, int NT, class S>
@@ -1026,20 +1034,32 @@ __device__ __forceinline__ void resolve_tile_kind(const ViewDesc& V, const Arena
     }
 }
 
+template <bool BATCH>
+__device__ __forceinline__ const ViewDesc& bin_view(const KArgs& ka, const ViewDesc* sview) {
+    if constexpr (BATCH) return *sview;
+    else return ka.one;
+}
+
+// sview (batches): the bin's view is copied into LDS once per bin, so that
+// every per-pixel read of its uniforms and pointers is an LDS read.
 template <int T, int NT, bool BATCH, int SK>
 __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uint32_t t, unsigned long long* keys,
-                                           uint32_t* bigq, uint32_t& nbig, uint16_t* plist) {
-    uint32_t vi = 0;
-    if constexpr (BATCH) vi = ka.tile_views[t];
-    const ViewDesc& V = view_at<BATCH>(ka, vi);
+                                           uint32_t* bigq, uint32_t& nbig, uint16_t* plist, ViewDesc* sview) {
+    if constexpr (BATCH) {
+        const uint32_t vi = ka.tile_views[t];
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(ka.table + vi);
+        for (uint32_t i = threadIdx.x; i < sizeof(ViewDesc) / 4; i += NT)
+            reinterpret_cast<uint32_t*>(sview)[i] = gld(src, i);
+    }
     const uint32_t n = ar.tile_count[t];
     const uint32_t off = ar.tile_off[t];
-    const uint32_t local = t - V.tile_base;
-    const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
-    const int ex = imin(ox + T, V.W), ey = imin(oy + T, V.H);
     for (int p = threadIdx.x; p < T * T; p += NT) keys[p] = EMPTY_KEY;
     if (threadIdx.x == 0) nbig = 0;
     __syncthreads();
+    const ViewDesc& V = bin_view<BATCH>(ka, sview);
+    const uint32_t local = t - V.tile_base;
+    const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
+    const int ex = imin(ox + T, V.W), ey = imin(oy + T, V.H);
     // every wave has read n: the count is consumed, leave it zero for the next frame
     if (threadIdx.x == 0) ar.tile_count[t] = 0;
 
@@ -1103,11 +1123,11 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
 constexpr uint32_t NUM_XCD = 8;
 
 // Six waves per SIMD (at most 80 VGPRs) for the shader kinds whose raster
-// kernel fits that without spilling: the batch's k_raster, ALU-bound, hides
-// more of its load latency with the extra wave (2.49 -> 2.43 ms).  Phong,
-// PBR and the mixed kernel would spill and keep the default.
+// kernel fits that without spilling; the normal-mapped shaders take the
+// registers they need (held to 80, the point-light one spilled 22 VGPRs and
+// ran 1.68 ms on the 256-view batch; unconstrained, 1.38 ms).
 constexpr int raster_min_waves(int sk) {
-    return (sk == SHADER_FLAT || sk == SHADER_GOURAUD || sk == SHADER_PHONG_POINT || sk == SHADER_FLAT_COLOR) ? 6 : 1;
+    return (sk == SHADER_FLAT || sk == SHADER_GOURAUD || sk == SHADER_FLAT_COLOR) ? 6 : 1;
 }
 
 template <int T, int NT, bool BATCH, int SK>
@@ -1116,6 +1136,7 @@ __global__ __launch_bounds__(NT, raster_min_waves(SK)) void k_raster(KArgs ka, A
     __shared__ uint32_t bigq[NT];
     __shared__ uint16_t plist[T * T];
     __shared__ uint32_t nbig;
+    __shared__ ViewDesc sview[1];
     if (ar.counters[CNT_ERROR]) return;
     // the pair overflow count was last needed by the scatter
     if (blockIdx.x == 0 && threadIdx.x == 0) ar.counters[CNT_OVF_PAIRS] = 0;
@@ -1124,7 +1145,7 @@ __global__ __launch_bounds__(NT, raster_min_waves(SK)) void k_raster(KArgs ka, A
     const uint32_t j = blockIdx.x / NUM_XCD;
     const uint32_t a = (blockIdx.x % NUM_XCD) * chunk + j;
     if (j >= chunk || a >= nactive) return;
-    raster_bin<T, NT, BATCH, SK>(ka, ar, ar.active[a], keys, bigq, nbig, plist);
+    raster_bin<T, NT, BATCH, SK>(ka, ar, ar.active[a], keys, bigq, nbig, plist, sview);
 }
 
 // ---------------------------------------------------------------------------
@@ -1183,17 +1204,17 @@ __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& 
     const uint32_t s = key >> 3, j = key & 7u;
     const DevGroup G = group_of(V, s);
     const uint32_t face = G.first + (s - G.rank_base);
-    const uint32_t* ip = V.idx + 3 * (size_t)face;
-    const uint32_t i0 = ip[0], i1 = ip[1], i2 = ip[2];
+    const uint3 ix = gld(reinterpret_cast<const uint3*>(V.idx), face);
+    const uint32_t i0 = ix.x, i1 = ix.y, i2 = ix.z;
     // the shaders' vertex stage reads the whole vertex; its first three
     // floats are the position (the packed copy setup read holds the same bits)
     constexpr bool kVerts = sizeof(typename S::Var) > 1;
     Vertex56 v0, v1, v2;
     vec3 p0, p1, p2;
-    if constexpr (kVerts) {
-        v0 = V.verts[i0];
-        v1 = V.verts[i1];
-        v2 = V.verts[i2];
+    if (kVerts && !V.vary) {
+        v0 = gld(V.verts, i0);
+        v1 = gld(V.verts, i1);
+        v2 = gld(V.verts, i2);
         p0 = make3(v0.px, v0.py, v0.pz);
         p1 = make3(v1.px, v1.py, v1.pz);
         p2 = make3(v2.px, v2.py, v2.pz);
@@ -1220,13 +1241,19 @@ __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& 
     if (!(z < dfb)) return false;   // reference mergeTileResults: strict less
     ShadeCtx c;
     c.u = &V.u;
-    c.m = &V.materials[G.material & REC_MATERIAL_MASK];
+    const DevMaterial mat = gld(V.materials, G.material & REC_MATERIAL_MASK);
+    c.m = &mat;
     c.face_colors = V.face_colors;
     c.prim = face;
     typename S::Var var[3];
     if constexpr (kVerts) {
         if (clipped) {
             for (int q = 0; q < 3; ++q) var[q] = S::vs(ar.clip_verts[3 * (size_t)slot + q], V.u);
+        } else if (V.vary) {
+            const typename S::Var* vy = static_cast<const typename S::Var*>(V.vary);
+            var[0] = gld(vy, i0);
+            var[1] = gld(vy, i1);
+            var[2] = gld(vy, i2);
         } else {
             var[0] = S::vs(v0, V.u);
             var[1] = S::vs(v1, V.u);
@@ -1245,9 +1272,9 @@ __device__ __forceinline__ bool resolve_vis_pixel(const ViewDesc& V, const Arena
     float z;
     uint32_t bgra;
     if (!shade_vis_pixel<S>(V, ar, k, px, py, dfb, z, bgra)) return false;
-    V.depth[pix] = z;
-    V.color[pix] = bgra;
-    if (V.ids) V.ids[pix] = (uint32_t)k;
+    gst(V.depth, pix, z);
+    gst(V.color, pix, bgra);
+    if (V.ids) gst(V.ids, pix, (uint32_t)k);
     return true;
 }
 
@@ -1259,9 +1286,9 @@ struct ClearSpec {
 };
 
 __device__ __forceinline__ void clear_pixel(const ViewDesc& V, size_t pix, const ClearSpec& cs) {
-    V.color[pix] = cs.bgra;
-    V.depth[pix] = cs.depth;
-    if (V.ids) V.ids[pix] = 0xFFFFFFFFu;
+    gst(V.color, pix, cs.bgra);
+    gst(V.depth, pix, cs.depth);
+    if (V.ids) gst(V.ids, pix, 0xFFFFFFFFu);
 }
 
 // Pixels p0, p0 + step, ... of the bin at (ox, oy), row-major.
@@ -1277,7 +1304,7 @@ __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena&
         bool drawn = false;
         if (k != EMPTY_KEY) {
             vk[pix] = EMPTY_KEY;
-            drawn = resolve_vis_pixel<S>(V, ar, k, px, py, pix, cs.on ? cs.depth : V.depth[pix]);
+            drawn = resolve_vis_pixel<S>(V, ar, k, px, py, pix, cs.on ? cs.depth : gld(V.depth, pix));
         }
         if (!drawn && cs.on) clear_pixel(V, pix, cs);
     }
@@ -1587,6 +1614,48 @@ __global__ void k_clear_views(const ViewDesc* views, uint32_t view0, uint32_t bg
 }
 
 // ---------------------------------------------------------------------------
+// k_vary: the vertex stage of every vertex of every job (a mesh shared by
+// views whose vertex stage reads the same uniforms), one lane per vertex;
+// the resolve then reads each winner's three outputs instead of running
+// the vertex stage per pixel.  The same function on the same inputs, so
+// the varyings are bit for bit those the resolve would compute.
+
+template <class S>
+__device__ __forceinline__ void vary_one(const VaryJob& j, uint32_t v, const ViewDesc& V) {
+    static_cast<typename S::Var*>(j.out)[v] = S::vs(j.verts[v], V.u);
+}
+
+__global__ __launch_bounds__(256) void k_vary(const VaryJob* jobs, const uint32_t* firsts, uint32_t njobs,
+                                              uint32_t total, const ViewDesc* views) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= total) return;
+    const VaryJob& j = jobs[find_le(firsts, njobs, g)];
+    const uint32_t v = g - j.first;
+    const ViewDesc& V = views[j.view];
+    switch (j.kind) {
+    case SHADER_FLAT: vary_one<FlatShader>(j, v, V); break;
+    case SHADER_PHONG: vary_one<PhongShader>(j, v, V); break;
+    case SHADER_PBR: vary_one<PBRShader>(j, v, V); break;
+    case SHADER_GOURAUD: vary_one<GouraudShader>(j, v, V); break;
+    case SHADER_PHONG_POINT: vary_one<PhongPointShader>(j, v, V); break;
+    default: break;   // FLAT_COLOR: no vertex stage
+    }
+}
+
+// Bytes of the vertex stage's output per vertex for a shader kind (0: the
+// shader has none).
+uint32_t vary_bytes(int kind) {
+    switch (kind) {
+    case SHADER_FLAT: return sizeof(FlatShader::Var);
+    case SHADER_PHONG: return sizeof(PhongShader::Var);
+    case SHADER_PBR: return sizeof(PBRShader::Var);
+    case SHADER_GOURAUD: return sizeof(GouraudShader::Var);
+    case SHADER_PHONG_POINT: return sizeof(PhongPointShader::Var);
+    default: return 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host-side launch wrappers
 
 // Expects a clean arena (bin counts and per-frame counters zero) and leaves
@@ -1664,8 +1733,9 @@ template <int T, bool BATCH>
 static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& wait) {
     const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
     if (vis_blocks) stage_launch1(k_vis<T, BATCH>, dim3(vis_blocks), dim3(256), st, ev, 0, ka, ar);
-    stage_launch1(k_clip<T, BATCH, true>, dim3(256), dim3(64), st, ev, 1, ka, ar);
-    stage_launch1(k_big<T, BATCH>, dim3(1024), dim3(256), st, ev, 2, ka, ar);
+    // passes the host showed to have no work are not launched
+    if (BATCH || !ka.skip_clip) stage_launch1(k_clip<T, BATCH, true>, dim3(256), dim3(64), st, ev, 1, ka, ar);
+    if (BATCH || !ka.skip_big) stage_launch1(k_big<T, BATCH>, dim3(1024), dim3(256), st, ev, 2, ka, ar);
     if (!BATCH && ka.two_pass && ka.clear && ka.shader >= 0 && ka.shader < SHADER_COUNT) {
         // shade at once, write the framebuffer after the earlier draws
         return ka.one.ids ? launch_two_pass<T, true>(ka, ar, st, ev, wait)
@@ -1689,7 +1759,7 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
     const uint32_t ntiles = ka.total_tiles;
     const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
     if (setup_blocks) stage_launch1(k_setup<T, BATCH>, dim3(setup_blocks), dim3(256), st, ev, 0, ka, ar);
-    stage_launch1(k_clip<T, BATCH, false>, dim3(256), dim3(64), st, ev, 1, ka, ar);
+    if (BATCH || !ka.skip_clip) stage_launch1(k_clip<T, BATCH, false>, dim3(256), dim3(64), st, ev, 1, ka, ar);
     if (ntiles <= SMALL_SCAN_TILES) {
         stage_launch1(k_scan_single, dim3(1), dim3(SCAN_THREADS), st, ev, 2, ar, ntiles);
     } else {
@@ -1727,6 +1797,11 @@ static hipError_t launch_frame_b(const KArgs& ka, const Arena& ar, hipStream_t s
 }
 
 hipError_t launch_frame(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& wait) {
+    // stage 5: a batch's vertex stage once per vertex of each shared mesh,
+    // before the resolve reads it
+    if (ka.table && ka.vary_total)
+        stage_launch1(k_vary, dim3((ka.vary_total + 255) / 256), dim3(256), st, ev, 5, ka.vary_jobs, ka.vary_firsts,
+                      ka.vary_njobs, ka.vary_total, ka.table);
     return ka.table ? launch_frame_b<true>(ka, ar, st, ev, wait) : launch_frame_b<false>(ka, ar, st, ev, wait);
 }
 

@@ -29,8 +29,14 @@ AXR_HD int cvt_i32(float f) {
     return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : INT32_MIN;
 }
 
+// Texel (x, y) with 0 <= x < w, 0 <= y < h; w * h <= 2^30, so the index
+// fits 32 bits.
+AXR_HD uint32_t texel_word(const DevTexture& t, int x, int y) {
+    return gld(t.texels, (uint32_t)y * (uint32_t)t.w + (uint32_t)x);
+}
+
 AXR_HD vec4 texel(const DevTexture& t, int x, int y) {
-    uint32_t p = t.texels[(size_t)y * (size_t)t.w + (size_t)x];
+    uint32_t p = texel_word(t, x, y);
     const float inv255 = 1.0f / 255.0f;
     return vec4{(float)(p & 255u) * inv255, (float)((p >> 8) & 255u) * inv255,
                 (float)((p >> 16) & 255u) * inv255, (float)(p >> 24) * inv255};
@@ -39,7 +45,7 @@ AXR_HD vec4 texel(const DevTexture& t, int x, int y) {
 // A texel's four bytes as floats 0..255 (the bilinear filter blends these
 // and scales by 1/255 once).
 AXR_HD vec4 texel_bytes(const DevTexture& t, int x, int y) {
-    uint32_t p = t.texels[(size_t)y * (size_t)t.w + (size_t)x];
+    uint32_t p = texel_word(t, x, y);
     return vec4{(float)(p & 255u), (float)((p >> 8) & 255u), (float)((p >> 16) & 255u), (float)(p >> 24)};
 }
 
@@ -287,7 +293,7 @@ struct FlatColorShader {
     struct Var { };
     static AXR_HD Var vs(const Vertex56&, const Uniforms&) { return Var{}; }
     static AXR_HD vec4 fs(vec3, const Var*, const ShadeCtx& c) {
-        uint32_t k = c.face_colors ? c.face_colors[c.prim] : 0xFFFFFFFFu;
+        uint32_t k = c.face_colors ? gld(c.face_colors, c.prim) : 0xFFFFFFFFu;
         const float inv255 = 1.0f / 255.0f;
         return vec4{((float)(k & 255u) + 0.5f) * inv255, ((float)((k >> 8) & 255u) + 0.5f) * inv255,
                     ((float)((k >> 16) & 255u) + 0.5f) * inv255, ((float)(k >> 24) + 0.5f) * inv255};

@@ -102,9 +102,9 @@ def stage_names(cfg, two_pass=False):
     two_pass: visibility-mode draws that resolve in two launches (an
     asynchronous draw with a clear)."""
     if cfg.get("raster_mode", 0) == 1:
-        return ("k_vis", "k_clip", "k_big", "k_shade", "k_finish") if two_pass else \
-            ("k_vis", "k_clip", "k_big", "none", "k_resolve")
-    return ("k_setup", "k_clip", "k_scan", "k_scatter", "k_raster")
+        return ("k_vis", "k_clip", "k_big", "k_shade", "k_finish", "k_vary") if two_pass else \
+            ("k_vis", "k_clip", "k_big", "none", "k_resolve", "k_vary")
+    return ("k_setup", "k_clip", "k_scan", "k_scatter", "k_raster", "k_vary")
 
 
 def stage_times(ctx, cfg, two_pass=False):
@@ -113,7 +113,8 @@ def stage_times(ctx, cfg, two_pass=False):
     dispatches (hipExtLaunchKernel), so they read what a kernel trace reads."""
     st = ctx.stats()
     n = max(1, st.frames)
-    return {name: st.stage_ms_total[i] / n for i, name in enumerate(stage_names(cfg, two_pass)) if name != "none"}
+    return {name: st.stage_ms_total[i] / n for i, name in enumerate(stage_names(cfg, two_pass))
+            if name != "none" and st.stage_ms_total[i] > 0}
 
 
 def compare_frames(gpu, ref, gpu_ids=None):
@@ -191,6 +192,7 @@ def run_frames(scene, steps, warmup, tile, use_tuned, workload="plain"):
     t0 = time.perf_counter()
     for _ in range(steps):
         frame()
+    t_enq = time.perf_counter() - t0
     ctx.sync()
     dt = time.perf_counter() - t0
     last = fb.read()
@@ -212,8 +214,8 @@ def run_frames(scene, steps, warmup, tile, use_tuned, workload="plain"):
     fb.close()
     gs.close()
     ctx.close()
-    return dict(seconds=dt, covered=covered, stages_ms=stages, mem_in_use=in_use, arena_bytes=arena, config=cfg,
-                last=last, checked=checked)
+    return dict(seconds=dt, enqueue_seconds=t_enq, covered=covered, stages_ms=stages, mem_in_use=in_use,
+                arena_bytes=arena, config=cfg, last=last, checked=checked)
 
 
 def frame_kernel_bytes(nverts, nfaces, W, H, covered, ntex):
@@ -300,7 +302,7 @@ def batch_kernel_bytes(wl, covered, ntex=2):
     covered pixel (the clear pass writes the rest of every view)."""
     verts = sum(wl.meshes[v.mesh][0].shape[0] for v in wl.views)
     return {"k_setup": 12 * verts + 12 * wl.total_faces, "k_raster": (8 + 4 * ntex) * covered,
-            "k_clear_views": 8 * wl.total_pixels}
+            "k_clear_views": 8 * wl.total_pixels, "k_vary": 0}
 
 
 SERVE_RATES = (100, 250, 500, 1000, 2500, 5000)
@@ -573,7 +575,8 @@ def main():
     nverts, nfaces = scene.vertices.shape[0], scene.nfaces
     W, H = scene.width, scene.height
     ntex = 2
-    log(f"[bench] plain path: {fps:.1f} frames/s ({ms:.4f} ms/frame), kernel ms {r['stages_ms']}")
+    log(f"[bench] plain path: {fps:.1f} frames/s ({ms:.4f} ms/frame, host enqueue "
+        f"{1e6 * r['enqueue_seconds'] / args.steps:.1f} us/frame), kernel ms {r['stages_ms']}")
 
     # roofline: each kernel's algorithmic bytes over its average duration
     # (events on its dispatches in the timed frames); the dominant kernel is
@@ -726,6 +729,7 @@ def main():
                                f"3 point lights, FP32 depth, BGRA8",
                    "tunables": r["config"], "covered_pixels": r["covered"]},
         "gtri_per_s": round(fps * nfaces / 1e9, 4),
+        "host_enqueue_us_per_frame": round(1e6 * r["enqueue_seconds"] / args.steps, 2),
         "frame_algorithmic_bytes": frame_bytes,
         "frame_roofline_frac": round(frame_bytes * fps / (HBM_PEAK_GBS * 1e9), 5),
         "kernels": kernels,

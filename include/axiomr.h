@@ -126,16 +126,25 @@ typedef struct axr_config {
  * resolve bin there; bin_capacity is unused. */
 enum { AXR_RASTER_BINNED = 0, AXR_RASTER_VISIBILITY = 1 };
 
-/* Stage timing collected with HIP events (recorded on the context's stream,
- * between the kernels) when profiling is on.  Binned mode: 0 setup
- * (transform, classification, cull, triangle setup, bin counts), 1 clip,
- * 2 prefix sum of the bin counts, 3 bin scatter, 4 per-bin raster +
- * resolve.  Visibility mode: 0 setup + small-triangle raster, 1 clip,
- * 2 large-triangle work items, 3 (empty), 4 resolve. */
+/* Kernel time per stage, from HIP events attached to the stage kernels'
+ * own dispatches, summed over the frames drawn while profiling is on
+ * (frames keep overlapping).  Binned mode: 0 setup (transform,
+ * classification, cull, triangle setup, bin counts), 1 clip, 2 prefix sum
+ * of the bin counts, 3 bin scatter, 4 per-bin raster + resolve.
+ * Visibility mode: 0 setup + small-triangle raster, 1 clip, 2 large-triangle
+ * work items, 3 shading of a two-pass resolve (else unused), 4 resolve or
+ * the two-pass framebuffer pass.  Both: 5 the batched form's vertex stage
+ * per mesh vertex. */
 typedef struct axr_kernel_stats {
     uint64_t frames;
     double frame_ms_total;
-    double stage_ms_total[5];
+    double stage_ms_total[6];
+    /* Counted whether or not profiling is on (reset with it): clipper and
+     * large-triangle passes not launched because the draw's bounds showed
+     * them empty, and draws in which such a pass turned out to have work
+     * (redrawn with every pass; none are skipped on that context after). */
+    uint64_t passes_skipped;
+    uint64_t skip_faults;
 } axr_kernel_stats;
 
 /* ---- context ---------------------------------------------------------- */
