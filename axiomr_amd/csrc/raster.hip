@@ -1034,6 +1034,96 @@ __device__ __forceinline__ void resolve_tile_kind(const ViewDesc& V, const Arena
     }
 }
 
+// ---------------------------------------------------------------------------
+// Coverage of a bin's small triangles.  One lane per triangle walks its
+// pixel rectangle (bbox within the bin), which leaves most lanes idle while
+// the wave waits for its largest rectangle: on the 256-view batch the
+// rectangles average 11.5 pixels but a wave's largest is far bigger, and
+// 42% of the lane slots did work.  So a lane walks at most WALK_CAP pixels
+// of its own rectangle; the rest of every longer rectangle is cut into
+// pieces of WALK_CAP pixels that the wave's lanes then take in turn.
+
+constexpr int WALK_CAP = 8;
+
+// Pixels [q0, q1) of the rectangle [x0, x0 + w) x [y0, ...) in row-major
+// order; every covered pixel takes the minimum of its key.
+template <int T>
+__device__ __forceinline__ void walk_span(const RasterTri& rt, int x0, int y0, int w, int q0, int q1, uint32_t key,
+                                          unsigned long long* keys, int ox, int oy, float inv_w) {
+    // row of q0 = floor(q0 / w): q0 < 2^16 and w <= 32, so (q0 + 0.5) / w is
+    // at least 1/64 from an integer and its rounded product within 2^-7 of it
+    const int dy = q0 == 0 ? 0 : (int)(((float)q0 + 0.5f) * inv_w);
+    int px = x0 + (q0 - dy * w), py = y0 + dy;
+    const int x1 = x0 + w;
+    for (int q = q0; q < q1; ++q) {
+        float al, be, ga, z;
+        if (cover(rt, px, py, al, be, ga, z) && z < INFINITY)
+            atomicMin(&keys[(py - oy) * T + (px - ox)], depth_key(z, key));
+        if (++px == x1) {
+            px = x0;
+            ++py;
+        }
+    }
+}
+
+// A wave's triangles whose rectangles outlast WALK_CAP pixels, for the
+// pieces: edge setup, rectangle and key per lane, and the first piece of
+// each lane in the wave's numbering.
+struct WalkPieces {
+    float4 e0[64];        // a0, b0, c0, a1
+    float4 e1[64];        // b1, c1, a2, b2
+    float4 e2[64];        // c2, inv_area, z0, z1
+    float4 e3[64];        // z2, bits(fx0), bits(x0 | y0 << 8 | w << 16 | h << 24) (bin coordinates), bits(key)
+    uint32_t start[64];
+};
+
+// All 64 lanes call it with their own triangle (w = h = 0 for none).
+template <int T>
+__device__ __forceinline__ void walk_wave(WalkPieces* wp, const RasterTri& rt, int x0, int y0, int w, int h,
+                                          uint32_t key, unsigned long long* keys, int ox, int oy) {
+    const int fp = w * h;
+    const float inv_w = w ? 1.0f / (float)w : 0.0f;
+    walk_span<T>(rt, x0 + ox, y0 + oy, w, 0, imin(fp, WALK_CAP), key, keys, ox, oy, inv_w);
+    const uint32_t pieces = fp > WALK_CAP ? (uint32_t)(fp - 1) / WALK_CAP : 0u;
+    const uint32_t incl = wave_incl_scan(pieces);
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0) return;
+    const uint32_t lane = __lane_id();
+    wp->start[lane] = incl - pieces;
+    if (pieces) {
+        wp->e0[lane] = make_float4(rt.a0, rt.b0, rt.c0, rt.a1);
+        wp->e1[lane] = make_float4(rt.b1, rt.c1, rt.a2, rt.b2);
+        wp->e2[lane] = make_float4(rt.c2, rt.inv_area, rt.z0, rt.z1);
+        wp->e3[lane] = make_float4(rt.z2, __int_as_float(rt.fx0),
+                                   __uint_as_float((uint32_t)x0 | ((uint32_t)y0 << 8) | ((uint32_t)w << 16) |
+                                                   ((uint32_t)h << 24)),
+                                   __uint_as_float(key));
+    }
+    // the wave's LDS writes before its reads of other lanes' entries
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t j = lane; j < total; j += 64u) {
+        // owner: the last lane whose pieces start at or before j (a lane
+        // without pieces shares its start with the next lane that has some)
+        uint32_t o = 0;
+        for (uint32_t step = 32; step; step >>= 1)
+            if (wp->start[o + step] <= j) o += step;
+        const float4 f0 = wp->e0[o], f1 = wp->e1[o], f2 = wp->e2[o], f3 = wp->e3[o];
+        RasterTri t;
+        t.a0 = f0.x; t.b0 = f0.y; t.c0 = f0.z; t.a1 = f0.w;
+        t.b1 = f1.x; t.c1 = f1.y; t.a2 = f1.z; t.b2 = f1.w;
+        t.c2 = f2.x; t.inv_area = f2.y; t.z0 = f2.z; t.z1 = f2.w;
+        t.z2 = f3.x; t.fx0 = __float_as_int(f3.y);
+        const uint32_t rect = __float_as_uint(f3.z);
+        const int rx = (int)(rect & 255u), ry = (int)((rect >> 8) & 255u);
+        const int rw = (int)((rect >> 16) & 255u), rh = (int)(rect >> 24);
+        const int q0 = WALK_CAP * (int)(j - wp->start[o] + 1);
+        walk_span<T>(t, rx + ox, ry + oy, rw, q0, imin(q0 + WALK_CAP, rw * rh), __float_as_uint(f3.w), keys, ox, oy,
+                     1.0f / (float)rw);
+    }
+}
+
 template <bool BATCH>
 __device__ __forceinline__ const ViewDesc& bin_view(const KArgs& ka, const ViewDesc* sview) {
     if constexpr (BATCH) return *sview;
@@ -1044,7 +1134,8 @@ __device__ __forceinline__ const ViewDesc& bin_view(const KArgs& ka, const ViewD
 // every per-pixel read of its uniforms and pointers is an LDS read.
 template <int T, int NT, bool BATCH, int SK>
 __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uint32_t t, unsigned long long* keys,
-                                           uint32_t* bigq, uint32_t& nbig, uint16_t* plist, ViewDesc* sview) {
+                                           uint32_t* bigq, uint32_t& nbig, uint16_t* plist, ViewDesc* sview,
+                                           WalkPieces* pieces) {
     if constexpr (BATCH) {
         const uint32_t vi = ka.tile_views[t];
         const uint32_t* src = reinterpret_cast<const uint32_t*>(ka.table + vi);
@@ -1063,31 +1154,30 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
     // every wave has read n: the count is consumed, leave it zero for the next frame
     if (threadIdx.x == 0) ar.tile_count[t] = 0;
 
+    WalkPieces* wp = pieces + (threadIdx.x >> 6);
     for (uint32_t b0 = 0; b0 < n; b0 += NT) {
         const uint32_t i = b0 + threadIdx.x;
+        RasterTri rt;
+        int x0 = 0, y0 = 0, w = 0, h = 0;
+        uint32_t key = 0;
         if (i < n) {
             const uint32_t ref = ar.bins[off + i];
             const TriRec r = rec_of(ar, ref);
-            RasterTri rt;
             if (raster_setup(r, rt)) {
-                const int x0 = imax(ox, rt.fx0), x1 = imin(ex, rt.cx1);
-                const int y0 = imax(oy, rt.fy0), y1 = imin(ey, rt.cy1);
-                const int w = x1 - x0, h = y1 - y0;
-                if (w > 0 && h > 0) {
-                    if (w * h <= ka.small_footprint) {
-                        const uint32_t key = r.c.y;
-                        for (int py = y0; py < y1; ++py)
-                            for (int px = x0; px < x1; ++px) {
-                                float al, be, ga, z;
-                                if (cover(rt, px, py, al, be, ga, z) && z < INFINITY)
-                                    atomicMin(&keys[(py - oy) * T + (px - ox)], depth_key(z, key));
-                            }
-                    } else {
-                        bigq[atomicAdd(&nbig, 1u)] = ref;
-                    }
+                x0 = imax(ox, rt.fx0);
+                y0 = imax(oy, rt.fy0);
+                w = imin(ex, rt.cx1) - x0;
+                h = imin(ey, rt.cy1) - y0;
+                if (w <= 0 || h <= 0) {
+                    w = h = 0;
+                } else if (w * h > ka.small_footprint) {
+                    bigq[atomicAdd(&nbig, 1u)] = ref;
+                    w = h = 0;
                 }
+                key = r.c.y;
             }
         }
+        walk_wave<T>(wp, rt, x0 - ox, y0 - oy, w, h, key, keys, ox, oy);
         __syncthreads();
         const uint32_t nb = nbig;
         for (uint32_t q = 0; q < nb; ++q) {
@@ -1137,6 +1227,7 @@ __global__ __launch_bounds__(NT, raster_min_waves(SK)) void k_raster(KArgs ka, A
     __shared__ uint16_t plist[T * T];
     __shared__ uint32_t nbig;
     __shared__ ViewDesc sview[1];
+    __shared__ WalkPieces pieces[NT / 64];
     if (ar.counters[CNT_ERROR]) return;
     // the pair overflow count was last needed by the scatter
     if (blockIdx.x == 0 && threadIdx.x == 0) ar.counters[CNT_OVF_PAIRS] = 0;
@@ -1145,7 +1236,7 @@ __global__ __launch_bounds__(NT, raster_min_waves(SK)) void k_raster(KArgs ka, A
     const uint32_t j = blockIdx.x / NUM_XCD;
     const uint32_t a = (blockIdx.x % NUM_XCD) * chunk + j;
     if (j >= chunk || a >= nactive) return;
-    raster_bin<T, NT, BATCH, SK>(ka, ar, ar.active[a], keys, bigq, nbig, plist, sview);
+    raster_bin<T, NT, BATCH, SK>(ka, ar, ar.active[a], keys, bigq, nbig, plist, sview, pieces);
 }
 
 // ---------------------------------------------------------------------------
