@@ -115,6 +115,20 @@ __device__ __forceinline__ void face_geo(const ViewDesc& V, uint32_t s, FaceGeo&
     f.c2 = V.mvp * make4(load_pos(V.pos, f.i2), 1.0f);
 }
 
+// Every 32-bit word of v as the wave's first active lane holds it: for a
+// value the whole wave holds in common, so the compiler keeps it in scalar
+// registers (and addresses through it with a scalar base).
+template <class X>
+__device__ __forceinline__ X wave_uniform(const X& v) {
+    static_assert(sizeof(X) % 4 == 0, "whole words");
+    X r;
+    uint32_t w[sizeof(X) / 4];
+    __builtin_memcpy(w, &v, sizeof(X));
+    for (int i = 0; i < (int)(sizeof(X) / 4); ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    __builtin_memcpy(&r, w, sizeof(X));
+    return r;
+}
+
 // Wave-aggregated append: returns this lane's slot for lanes with pred set.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     unsigned long long mask = __ballot(pred);
@@ -913,13 +927,17 @@ __device__ __forceinline__ const TriRec& rec_of(const Arena& ar, uint32_t ref) {
 // Vertex stage of the winner's three corners, then its fragment stage
 // (reference tiled_pipeline.cpp:185-212 vertex, 564-575 fragment).  Clipped
 // pieces take their corners from the clip records' vertices.
+// um: the material every face of the view uses, held in scalar registers
+// (a single-group mesh), or null: the record's material is loaded.
 template <class S>
 __device__ __forceinline__ vec4 shade_at(const ViewDesc& V, const Arena& ar, uint32_t material, uint32_t prim,
                                          bool clipped, uint32_t slot, uint32_t i0, uint32_t i1, uint32_t i2,
-                                         vec3 bar) {
+                                         vec3 bar, const DevMaterial* um = nullptr) {
     ShadeCtx c;
     c.u = &V.u;
-    const DevMaterial mat = gld(V.materials, material & REC_MATERIAL_MASK);
+    DevMaterial mat;
+    if (um) mat = *um;
+    else mat = gld(V.materials, material & REC_MATERIAL_MASK);
     c.m = &mat;
     c.face_colors = V.face_colors;
     c.prim = prim;
@@ -943,7 +961,7 @@ __device__ __forceinline__ vec4 shade_at(const ViewDesc& V, const Arena& ar, uin
 
 template <class S>
 __device__ __forceinline__ vec4 shade_winner(const ViewDesc& V, const Arena& ar, const TriRec& r, uint32_t slot,
-                                            bool clipped, uint32_t rank, vec3 bar) {
+                                            bool clipped, uint32_t rank, vec3 bar, const DevMaterial* um = nullptr) {
     uint32_t prim, i0 = 0, i1 = 0, i2 = 0;
     if (clipped) {
         const DevGroup G = group_of(V, rank);
@@ -957,14 +975,14 @@ __device__ __forceinline__ vec4 shade_winner(const ViewDesc& V, const Arena& ar,
             i2 = ix.z;
         }
     }
-    return shade_at<S>(V, ar, r.c.w, prim, clipped, slot, i0, i1, i2, bar);
+    return shade_at<S>(V, ar, r.c.w, prim, clipped, slot, i0, i1, i2, bar, um);
 }
 
 // known: the depth plane is known to hold known_depth here (a clear written
 // just before the draw); else it is loaded.
 template <class S>
 __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px, int py,
-                                              bool known, float known_depth) {
+                                              bool known, float known_depth, const DevMaterial* um) {
     const uint32_t key = (uint32_t)k;
     const uint32_t rank = key >> 3, j = key & 7u;
     TriRec r = ar.rec[V.face_base + rank];
@@ -981,7 +999,7 @@ __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar
     cover(rt, px, py, al, be, ga, z);
     const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
     if (!(z < (known ? known_depth : gld(V.depth, pix)))) return;   // reference mergeTileResults: strict less
-    const vec4 col = shade_winner<S>(V, ar, r, slot, clipped, rank, make3(al, be, ga));
+    const vec4 col = shade_winner<S>(V, ar, r, slot, clipped, rank, make3(al, be, ga), um);
     gst(V.depth, pix, z);
     gst(V.color, pix, pack_bgra(col));
     if (V.ids) gst(V.ids, pix, key);
@@ -1002,9 +1020,19 @@ __device__ __forceinline__ void resolve_tile(const ViewDesc& V, const Arena& ar,
     }
     __syncthreads();
     const uint32_t n = *ncov;
-    for (uint32_t i = threadIdx.x; i < n; i += NT) {
-        const int p = plist[i];
-        resolve_pixel<S>(V, ar, keys[p], ox + (p % T), oy + (p / T), known, known_depth);
+    if (V.ngroups == 1) {
+        // one material for the whole view: its fields (texture pointers
+        // among them) in scalar registers, loaded once
+        const DevMaterial um = wave_uniform(gld(V.materials, V.g0.material & REC_MATERIAL_MASK));
+        for (uint32_t i = threadIdx.x; i < n; i += NT) {
+            const int p = plist[i];
+            resolve_pixel<S>(V, ar, keys[p], ox + (p % T), oy + (p / T), known, known_depth, &um);
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < n; i += NT) {
+            const int p = plist[i];
+            resolve_pixel<S>(V, ar, keys[p], ox + (p % T), oy + (p / T), known, known_depth, nullptr);
+        }
     }
 }
 
@@ -1188,8 +1216,11 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
             const int y0 = imax(oy, rt.fy0), y1 = imin(ey, rt.cy1);
             const int w = x1 - x0, np = w * (y1 - y0);
             const uint32_t key = r.c.y;
+            const float inv_w = 1.0f / (float)w;
             for (int p = threadIdx.x; p < np; p += NT) {
-                const int px = x0 + p % w, py = y0 + p / w;
+                // p / w as in walk_span (p < 2^16, w <= 32)
+                const int dy = (int)(((float)p + 0.5f) * inv_w);
+                const int px = x0 + (p - dy * w), py = y0 + dy;
                 float al, be, ga, z;
                 if (cover(rt, px, py, al, be, ga, z) && z < INFINITY)
                     atomicMin(&keys[(py - oy) * T + (px - ox)], depth_key(z, key));

@@ -32,7 +32,10 @@ AXR_HD int cvt_i32(float f) {
 // Texel (x, y) with 0 <= x < w, 0 <= y < h; w * h <= 2^30, so the index
 // fits 32 bits.
 AXR_HD uint32_t texel_word(const DevTexture& t, int x, int y) {
-    return gld(t.texels, (uint32_t)y * (uint32_t)t.w + (uint32_t)x);
+    // a 32-bit byte offset: with the texel pointer in scalar registers the
+    // load takes a scalar base and a 32-bit vector offset
+    const uint32_t off = ((uint32_t)y * (uint32_t)t.w + (uint32_t)x) * 4u;
+    return gld(reinterpret_cast<const uint32_t*>(reinterpret_cast<const unsigned char*>(t.texels) + off), 0u);
 }
 
 AXR_HD vec4 texel(const DevTexture& t, int x, int y) {
