@@ -358,21 +358,29 @@ __device__ __forceinline__ void emit_pairs_wave(const Arena& ar, const ViewDesc&
     if (__lane_id() == 0) ar.wave_pairs[wave] = total > ar.wave_cap ? 0u : total;
     if (total == 0) return;
     const uint32_t p0 = base + incl - n;
+    const uint32_t lane = __lane_id();
     for (uint32_t k = 0; __ballot(k < n); ++k) {
         const bool act = k < n;
         const uint32_t bin = act ? br.bin(k, V) : 0u;
-        unsigned long long pending = __ballot(act);
+        // group the lanes by bin (register work only), then every group's
+        // first lane takes the group's slots with one atomic, all groups'
+        // atomics in flight together
+        unsigned long long pending = __ballot(act), mine = 0ull;
+        int leader_of_mine = 0;
         while (pending) {
             const int leader = __ffsll((long long)pending) - 1;
-            const uint32_t lb = __shfl(bin, leader);
+            const uint32_t lb = __builtin_amdgcn_readlane(bin, leader);
             const unsigned long long m = __ballot(act && bin == lb) & pending;
-            uint32_t slot = 0;
-            if ((int)__lane_id() == leader) slot = atomicAdd(&ar.tile_count[lb], (uint32_t)__popcll(m));
-            slot = __shfl(slot, leader);
-            if (((m >> __lane_id()) & 1ull) && fits)
-                ar.pairs[p0 + k] = make_uint4(lb, slot + (uint32_t)__popcll(m & __lanemask_lt()), ref, 0u);
+            if ((m >> lane) & 1ull) {
+                mine = m;
+                leader_of_mine = leader;
+            }
             pending &= ~m;
         }
+        uint32_t slot = 0;
+        if (act && leader_of_mine == (int)lane) slot = atomicAdd(&ar.tile_count[bin], (uint32_t)__popcll(mine));
+        slot = __shfl(slot, leader_of_mine);
+        if (act && fits) ar.pairs[p0 + k] = make_uint4(bin, slot + (uint32_t)__popcll(mine & __lanemask_lt()), ref, 0u);
     }
 }
 
