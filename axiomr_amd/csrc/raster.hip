@@ -477,17 +477,13 @@ __device__ __forceinline__ void big_append_lane(const Arena& ar, const TriRec& r
 // ---------------------------------------------------------------------------
 // k_setup: one lane per face (submission order across views).
 
-template <int T, bool BATCH>
-__global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
-    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
-    if (ar.counters[CNT_ERROR]) return;
+// Face slot g of view V (all 64 lanes of the wave call it).
+template <int T>
+__device__ __forceinline__ void setup_face(const KArgs& ka, const Arena& ar, uint32_t g, const ViewDesc& V) {
     bool active = g < ka.total_faces;
     bool want_clip = false, binnable = false;
     Rect rc{0, 0, 0, 0};
-    uint32_t vi = 0;
-    if constexpr (BATCH) vi = wave_view_of(ka.face_bases, ka.nviews, g, ka.total_faces);
     if (active) {
-        const ViewDesc& V = view_at<BATCH>(ka, vi);
         const uint32_t s = g - V.face_base;
         FaceGeo f;
         face_geo(V, s, f);
@@ -516,7 +512,27 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
         else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
         if (ka.skip_clip) atomicOr(&ar.counters[CNT_ERROR], ERR_SKIPPED);
     }
-    emit_pairs_wave<T>(ar, view_at<BATCH>(ka, vi), binnable, rc, g, g >> 6);
+    emit_pairs_wave<T>(ar, V, binnable, rc, g, g >> 6);
+}
+
+template <int T, bool BATCH>
+__global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (ar.counters[CNT_ERROR]) return;
+    if constexpr (BATCH) {
+        const uint32_t vi = wave_view_of(ka.face_bases, ka.nviews, g, ka.total_faces);
+        const uint32_t v0 = __builtin_amdgcn_readfirstlane(vi);
+        if (__ballot(vi != v0) == 0ull) {
+            // the whole wave in one view (the usual case): the view's fields
+            // in scalar registers, loaded once for the wave
+            const ViewDesc V = wave_uniform(ka.table[v0]);
+            setup_face<T>(ka, ar, g, V);
+        } else {
+            setup_face<T>(ka, ar, g, ka.table[vi]);
+        }
+    } else {
+        setup_face<T>(ka, ar, g, ka.one);
+    }
 }
 
 // ---------------------------------------------------------------------------
