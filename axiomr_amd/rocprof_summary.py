@@ -3,6 +3,10 @@
     python -m axiomr_amd.rocprof_summary stats DIR [--title T] [--command C]
         Markdown table of kernel_stats.csv (calls, average/min/max duration).
 
+    python -m axiomr_amd.rocprof_summary sq DIR [--stats DIR]
+        Per-kernel lane utilisation, VALU instructions per wave and VALU busy
+        from one --pmc pass of SQ_COUNTERS (and the durations of a stats run).
+
     python -m axiomr_amd.rocprof_summary traffic --fetch DIR --write DIR \
         --copy-fetch DIR --copy-write DIR --copy-bytes N [--tunables JSON]
         [--workload W] [--command C ...] [--out JSON]
@@ -119,6 +123,50 @@ def traffic(fetch_dir: Path, write_dir: Path, copy_fetch: Path, copy_write: Path
     return {"copy_fetch": cf, "copy_write": cw, "kernels": kernels}
 
 
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES",
+               "SQ_BUSY_CYCLES", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT")
+
+
+def sq_table(sq_dir: Path, stats_dir: Path, clock_ghz: float = 2.4, simds: int = 1024) -> List[Dict]:
+    """Per kernel, from one --pmc pass of SQ_COUNTERS (medians over the
+    kernel's dispatches, whole device) and the kernel's average duration
+    from a --kernel-trace --stats run of the same command:
+      lane utilisation = SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU)
+          (the share of the 64 lanes that did work in the VALU cycles),
+      VALU busy = 4 x SQ_ACTIVE_INST_VALU / (SIMDs x kernel cycles),
+      VALU instructions per wave = SQ_INSTS_VALU / SQ_WAVES."""
+    per = {c: counter_per_kernel(sq_dir, c) for c in SQ_COUNTERS}
+    dur = {r["name"]: r["avg_ns"] for r in kernel_stats(stats_dir)} if stats_dir else {}
+    rows = []
+    for k in sorted(per["SQ_WAVES"], key=lambda n: -per["SQ_INSTS_VALU"].get(n, {}).get("median", 0.0)):
+        m = {c: per[c].get(k, {}).get("median", 0.0) for c in SQ_COUNTERS}
+        act = m["SQ_ACTIVE_INST_VALU"]
+        ns = dur.get(k)
+        rows.append({
+            "kernel": k, "waves": m["SQ_WAVES"],
+            "valu_per_wave": m["SQ_INSTS_VALU"] / m["SQ_WAVES"] if m["SQ_WAVES"] else 0.0,
+            "lane_util": m["SQ_THREAD_CYCLES_VALU"] / (64.0 * act) if act else 0.0,
+            "valu_busy": 4.0 * act / (simds * ns * clock_ghz) if ns else None,
+            "avg_us": ns / 1e3 if ns else None,
+            "lds_insts": m["SQ_INSTS_LDS"], "lds_bank_conflicts": m["SQ_LDS_BANK_CONFLICT"],
+            "counters": m,
+        })
+    return rows
+
+
+def sq_markdown(rows: List[Dict]) -> str:
+    out = ["| kernel | avg us | waves | VALU inst / wave | lane utilisation | VALU busy | LDS inst | LDS bank-conflict cycles |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        if not r["counters"]["SQ_INSTS_VALU"]:
+            continue
+        us = f"{r['avg_us']:.2f}" if r["avg_us"] is not None else "-"
+        busy = f"{100 * r['valu_busy']:.1f} %" if r["valu_busy"] is not None else "-"
+        out.append(f"| `{r['kernel']}` | {us} | {r['waves']:.0f} | {r['valu_per_wave']:.0f} | "
+                   f"{r['lane_util']:.2f} | {busy} | {r['lds_insts']:.3g} | {r['lds_bank_conflicts']:.3g} |")
+    return "\n".join(out) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -134,9 +182,14 @@ def main(argv=None):
     t.add_argument("--workload", default="")
     t.add_argument("--command", action="append", default=[], help="profiling commands, recorded verbatim")
     t.add_argument("--out", default="")
+    q = sub.add_parser("sq")
+    q.add_argument("dir", help="output of one --pmc pass of SQ_COUNTERS")
+    q.add_argument("--stats", default="", help="output of a --kernel-trace --stats run of the same command")
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         sys.stdout.write(stats_markdown(Path(args.dir), args.title, args.command))
+    elif args.cmd == "sq":
+        sys.stdout.write(sq_markdown(sq_table(Path(args.dir), Path(args.stats) if args.stats else None)))
     else:
         res = traffic(Path(args.fetch), Path(args.write), Path(args.copy_fetch), Path(args.copy_write),
                       args.copy_bytes)

@@ -33,6 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE) on this workload, scaled by a plain-copy calibration; written
 # by `python -m axiomr_amd.rocprof_summary traffic`.
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_icosphere1080p.json"
+BATCH_TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_batch256.json"
 
 
 def log(msg):
@@ -497,17 +498,20 @@ def cpu_baseline(scene, budget_s=12.0, threads=16, min_frames=1, max_frames=50):
                 phases_s={k: v / len(times) for k, v in phases.items()})
 
 
-def measured_traffic(kname, tunables):
-    """HBM bytes per launch of `kname` from the committed --pmc summary, if
-    it was taken with the same tunables (else None)."""
-    if not TRAFFIC_FILE.exists():
+def measured_traffic(kname, tunables, path=TRAFFIC_FILE, instance=""):
+    """HBM bytes per launch of `kname` from a committed --pmc summary, if it
+    was taken with the same tunables (else None); `instance`: a substring of
+    the template arguments that picks one instantiation ("true": the
+    batch's, "false": the plain frame's without an id plane)."""
+    if not path.exists():
         return None
-    data = json.loads(TRAFFIC_FILE.read_text())
+    data = json.loads(path.read_text())
     if data.get("tunables") != tunables:
         return None
     total = 0.0
     for part in kname.split("+"):   # "k_shade+k_finish": the sum of both launches
-        hits = [v for k, v in data.get("kernels", {}).items() if k.startswith(part + "<") or k == part]
+        hits = [v for k, v in data.get("kernels", {}).items()
+                if (k.startswith(part + "<") and instance in k) or (k == part and not instance)]
         if len(hits) != 1:
             return None
         total += hits[0]["traffic_bytes"]
@@ -673,7 +677,8 @@ def main():
                  "batch_ms": round(b["batch_ms"], 4), "kernel_ms": {k: round(v, 5) for k, v in b["stages_ms"].items()},
                  "roofline": {"bound": "hbm", "kernel": bdom, "achieved": round(bach, 2), "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": round(bach / HBM_PEAK_GBS, 5),
-                              "bytes_per_launch": bkb.get(bdom, 0), "avg_launch_ms": round(bms, 5)},
+                              "bytes_per_launch": bkb.get(bdom, 0), "avg_launch_ms": round(bms, 5),
+                              "traffic": measured_traffic(bdom, b["config"], BATCH_TRAFFIC_FILE, "true")},
                  "batch_algorithmic_bytes": sum(bkb.values()),
                  "batch_roofline_frac": round(sum(bkb.values()) / (b["batch_ms"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 5),
                  "tunables": b["config"], "plain_tunables": b["plain_config"],
@@ -735,7 +740,7 @@ def main():
         "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic(kname, r["config"]),
+                     "traffic": measured_traffic(kname, r["config"], TRAFFIC_FILE, "false"),
                      "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 6)},
         "cpu_baseline": cpu,
         "parity": parity,
