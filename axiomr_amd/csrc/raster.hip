@@ -921,15 +921,35 @@ __global__ __launch_bounds__(256) void k_scatter(Arena ar, uint32_t nwaves) {
     const uint32_t stride = gridDim.x * 256u;
     const uint32_t cap = ar.wave_cap;
     // one wave per setup wave's region: its pair count is read once and
-    // only the pairs it holds are visited
+    // only the pairs it holds are visited.  A wave takes SCATTER_REGIONS
+    // regions at a time with their loads in flight together (each region's
+    // chain is count -> pair -> bin offset -> store; a region rarely holds
+    // more than 64 pairs)
+    constexpr uint32_t SCATTER_REGIONS = 4;
     const uint32_t lane = __lane_id();
     const uint32_t nw = gridDim.x * 4u;
-    for (uint32_t w = (blockIdx.x * 256u + threadIdx.x) >> 6; w < nwaves; w += nw) {
-        const uint32_t n = ar.wave_pairs[w];
-        const uint4* reg = ar.pairs + (size_t)w * cap;
-        for (uint32_t i = lane; i < n; i += 64u) {
-            const uint4 p = reg[i];
-            ar.bins[ar.tile_off[p.x] + p.y] = p.z;
+    for (uint32_t w0 = (blockIdx.x * 256u + threadIdx.x) >> 6; w0 < nwaves; w0 += nw * SCATTER_REGIONS) {
+        uint32_t n[SCATTER_REGIONS], off[SCATTER_REGIONS];
+        uint4 p[SCATTER_REGIONS];
+#pragma unroll
+        for (uint32_t k = 0; k < SCATTER_REGIONS; ++k) {
+            const uint32_t w = w0 + k * nw;
+            n[k] = w < nwaves ? ar.wave_pairs[w] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < SCATTER_REGIONS; ++k)
+            if (lane < n[k]) p[k] = ar.pairs[(size_t)(w0 + k * nw) * cap + lane];
+#pragma unroll
+        for (uint32_t k = 0; k < SCATTER_REGIONS; ++k)
+            if (lane < n[k]) off[k] = ar.tile_off[p[k].x];
+#pragma unroll
+        for (uint32_t k = 0; k < SCATTER_REGIONS; ++k) {
+            if (lane < n[k]) ar.bins[off[k] + p[k].y] = p[k].z;
+            const uint4* reg = ar.pairs + (size_t)(w0 + k * nw) * cap;
+            for (uint32_t i = lane + 64u; i < n[k]; i += 64u) {
+                const uint4 q = reg[i];
+                ar.bins[ar.tile_off[q.x] + q.y] = q.z;
+            }
         }
     }
     const uint32_t novf = imin((int)ar.counters[CNT_OVF_PAIRS], (int)ar.cap_ovf);
