@@ -19,6 +19,8 @@
 // The per-pixel winner is the minimum of the 64-bit key
 // (ordered depth, submission key), which equals the reference's first-wins
 // strict-less depth test over triangles in submission order.
+#include <cstdlib>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include "axr_types.hpp"
 #include "shade.hpp"
 #include "internal.hpp"
@@ -1828,12 +1830,37 @@ uint32_t vary_bytes(int kind) {
 // it clean.
 constexpr int RESOLVE_MAX_BLOCKS = 16384;
 
+// Tool ranges named after the reference's phases (its Tracy zones,
+// tiled_pipeline.cpp:173, 264, 268, 273, 283, 312) around each stage's
+// launches, for a profiler's marker trace; on when AXR_ROCTX=1 at library
+// load.  The stages of a raster mode map onto the phases as in
+// kBinnedPhases / kVisPhases (the vertex pass of a batch belongs to the
+// reference's raster phase, where its fragment shading runs).
+static bool phase_ranges_on() {
+    static const bool on = [] {
+        const char* v = getenv("AXR_ROCTX");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
+static const char* const kBinnedPhases[kStages] = {"ClippingAllTriangles", "ClippingAllTriangles", "TilePreparation",
+                                                   "BinAllTriangles", "RasterizingTiles", "RasterizingTiles"};
+static const char* const kVisPhases[kStages] = {"ClippingAllTriangles", "ClippingAllTriangles", "RasterizingTiles",
+                                                "RasterizingTiles", "MergingAllTiles", "RasterizingTiles"};
+static thread_local const char* const* t_phases = kBinnedPhases;
+
 // One kernel of stage `stage`.  With kernel timing (ev), the first kernel
 // of a stage carries the stage's start event and its last kernel the stop
 // event, attached to the dispatches (hipExtLaunchKernel).
 template <class... KA, class... A>
 static void stage_launch(void (*kern)(KA...), dim3 grid, dim3 block, hipStream_t st, StageEvents* ev, int stage,
                          bool first, bool last, A... args) {
+    const bool ranges = phase_ranges_on();
+    if (ranges && first) roctxRangePush(t_phases[stage]);
+    struct Pop {
+        bool on;
+        ~Pop() { if (on) roctxRangePop(); }
+    } pop{ranges && last};
     if (ev) {
         hipExtLaunchKernelGGL(kern, grid, block, 0, st, first ? ev->start[stage] : nullptr,
                               last ? ev->stop[stage] : nullptr, 0, args...);
@@ -1897,6 +1924,7 @@ static hipError_t launch_two_pass(const KArgs& ka, const Arena& ar, hipStream_t 
 // k_resolve or (two-pass) 3 k_shade and 4 k_finish.
 template <int T, bool BATCH>
 static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& wait) {
+    t_phases = kVisPhases;
     const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
     if (vis_blocks) stage_launch1(k_vis<T, BATCH>, dim3(vis_blocks), dim3(256), st, ev, 0, ka, ar);
     // passes the host showed to have no work are not launched
@@ -1922,6 +1950,7 @@ template <int T, int NT, bool BATCH>
 static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev,
                                  const FbWait& wait) {
     if (ka.mode == RASTER_VISIBILITY) return launch_vis_t<T, BATCH>(ka, ar, st, ev, wait);
+    t_phases = kBinnedPhases;
     const uint32_t ntiles = ka.total_tiles;
     const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
     if (setup_blocks) stage_launch1(k_setup<T, BATCH>, dim3(setup_blocks), dim3(256), st, ev, 0, ka, ar);

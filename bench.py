@@ -498,11 +498,12 @@ def cpu_baseline(scene, budget_s=12.0, threads=16, min_frames=1, max_frames=50):
                 phases_s={k: v / len(times) for k, v in phases.items()})
 
 
-def measured_traffic(kname, tunables, path=TRAFFIC_FILE, instance=""):
+def measured_traffic(kname, tunables, path=None, instance=""):
     """HBM bytes per launch of `kname` from a committed --pmc summary, if it
     was taken with the same tunables (else None); `instance`: a substring of
     the template arguments that picks one instantiation ("true": the
     batch's, "false": the plain frame's without an id plane)."""
+    path = TRAFFIC_FILE if path is None else path
     if not path.exists():
         return None
     data = json.loads(path.read_text())
@@ -740,7 +741,7 @@ def main():
         "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic(kname, r["config"], TRAFFIC_FILE, "false"),
+                     "traffic": measured_traffic(kname, r["config"], instance="false"),
                      "bytes_per_launch": kbytes, "avg_launch_ms": round(kms, 6)},
         "cpu_baseline": cpu,
         "parity": parity,

@@ -32,6 +32,19 @@ def test_traffic_of_a_launch_pair(tmp_path, monkeypatch):
     assert bench.measured_traffic("k_shade+k_finish4", {**TUN, "tile_size": 8}) is None
 
 
+def test_traffic_of_one_instantiation(tmp_path, monkeypatch):
+    # the batch's kernels and the plain frame's share names; the template
+    # argument picks one
+    f = _traffic_file(tmp_path, {"k_raster<32,128,true,4>": 7.0, "k_raster<32,128,false,4>": 3.0,
+                                 "k_shade<16,4,false>": 2.0, "k_shade<16,4,true>": 9.0})
+    monkeypatch.setattr(bench, "TRAFFIC_FILE", f)
+    assert bench.measured_traffic("k_raster", TUN) is None
+    assert bench.measured_traffic("k_raster", TUN, instance="true") == 7
+    assert bench.measured_traffic("k_shade", TUN, instance="false") == 2
+    assert bench.measured_traffic("k_raster", TUN, path=f, instance="false") == 3
+    assert bench.measured_traffic("k_raster", TUN, path=tmp_path / "absent.json", instance="true") is None
+
+
 def test_traffic_needs_one_match_per_part(tmp_path, monkeypatch):
     # "k_finish<" must not also match "k_finish4<"; two instantiations of one
     # name are ambiguous
