@@ -161,6 +161,9 @@ enum RasterMode : int32_t {
 // Device frame arena: the per-frame working memory of the binning pipeline.
 struct Arena {
     uint32_t* counters;        // CNT_COUNT words
+    uint32_t* host_err;        // pinned host word set nonzero with every error bit, so a
+                               // host wait learns of an error without copying the
+                               // counters back (null: no such word)
     TriRec* rec;               // binned mode: one record per face slot of every view
     uint32_t* clip_base;       // visibility mode (same memory as rec): first clip
                                // record of each clipped face slot

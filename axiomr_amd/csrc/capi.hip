@@ -179,6 +179,7 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs
         (r = grow(ctx, err, fa, fa.vcol, fs.color_slots * 4)))
         return r;
     a.counters = (uint32_t*)fa.counters.p;
+    a.host_err = fa.host_err;
     a.rec = (TriRec*)fa.rec.p;
     a.clip_base = (uint32_t*)fa.rec.p;
     a.clip_faces = (uint32_t*)fa.clip_faces.p;
@@ -218,6 +219,7 @@ int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs
 // captured graphs.
 static int arena_reset(ErrSink& err, FrameArena& fa) {
     std::memset(fa.host_counters, 0, CNT_COUNT * 4);
+    *fa.host_err = 0;
     hipError_t e = hipMemcpyAsync(fa.counters.p, fa.host_counters, CNT_COUNT * 4, hipMemcpyHostToDevice, fa.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(fa.stream);
     fa.dirty = true;
@@ -271,10 +273,16 @@ void sig_release(int device, uint64_t* p) {
 }
 }  // namespace
 
+// The pinned block of an arena: the counters as last copied back, then the
+// error word the kernels set, 128-byte aligned past them.
+constexpr uint32_t HOST_ERR_WORD = (CNT_COUNT + 31) / 32 * 32 + 32;
+
 int arena_create(axr_ctx* ctx, ErrSink& err, FrameArena& fa) {
     hipError_t e = hipStreamCreateWithFlags(&fa.stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&fa.host_counters, CNT_COUNT * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&fa.host_counters, (HOST_ERR_WORD + 32) * 4, hipHostMallocDefault);
     if (e != hipSuccess) return hip_fail(err, e, "frame arena stream");
+    fa.host_err = fa.host_counters + HOST_ERR_WORD;
+    *fa.host_err = 0;
     FrameSize fs;
     fs.faces = 1024;
     fs.tiles = 1024;
@@ -321,6 +329,16 @@ int arena_read_counters(ErrSink& err, FrameArena& fa) {
     hipError_t e = hipMemcpyAsync(fa.host_counters, fa.counters.p, CNT_COUNT * 4, hipMemcpyDeviceToHost, fa.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(fa.stream);
     return e == hipSuccess ? AXR_OK : hip_fail(err, e, "read counters");
+}
+
+// Every device error bit also sets the pinned flag, so without the flag the
+// device counters hold no error.
+int arena_counters_after_wait(ErrSink& err, FrameArena& fa) {
+    if (__atomic_load_n(fa.host_err, __ATOMIC_ACQUIRE) == 0) {
+        fa.host_counters[CNT_ERROR] = 0;
+        return AXR_OK;
+    }
+    return arena_read_counters(err, fa);
 }
 
 static int arena_prep_if_dirty(ErrSink& err, FrameArena& fa) {
@@ -582,8 +600,12 @@ int axr_ctx_create(int device, uint64_t hbm_budget_bytes, axr_ctx** out) {
     ctx->cfg.bin_capacity = 128;
     ctx->cfg.small_footprint = 12;
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return AXR_ERR_HIP; }
-    for (auto& e : ctx->join_ev)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { axr_ctx_destroy(ctx); return AXR_ERR_HIP; }
+    for (auto* evs : {ctx->join_ev, ctx->sync_ev})
+        for (int k = 0; k < axr_ctx::kArenas; ++k)
+            if (hipEventCreateWithFlags(&evs[k], hipEventDisableTiming) != hipSuccess) {
+                axr_ctx_destroy(ctx);
+                return AXR_ERR_HIP;
+            }
     if (hipHostMalloc((void**)&ctx->noalloc_counters, CNT_COUNT * 4, hipHostMallocDefault) != hipSuccess) {
         axr_ctx_destroy(ctx);
         return AXR_ERR_OOM;
@@ -602,8 +624,9 @@ int axr_ctx_destroy(axr_ctx* ctx) {
     if (ctx->main.stream) (void)ctx_sync_all(ctx);
     for (auto& a : ctx->alt) arena_destroy(ctx, a);
     arena_destroy(ctx, ctx->main);
-    for (auto& e : ctx->join_ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto* evs : {ctx->join_ev, ctx->sync_ev})
+        for (int k = 0; k < axr_ctx::kArenas; ++k)
+            if (evs[k]) (void)hipEventDestroy(evs[k]);
     if (ctx->noalloc_counters) (void)hipHostFree(ctx->noalloc_counters);
     if (ctx->ev) {
         for (int i = 0; i < axr_ctx::kEventRing; ++i)
@@ -677,11 +700,11 @@ int axr_ctx_memory(const axr_ctx* ctx, uint64_t* in_use, uint64_t* arena) {
     return AXR_OK;
 }
 
-// After a host wait: an arena whose frames overflowed is grown to what the
-// kernels reported (the current capacity is the floor) and reset.
+// After a host wait and arena_counters_after_wait: an arena whose frames
+// overflowed is grown to what the kernels reported (the current capacity is
+// the floor) and reset.
 static int arena_settle_overflow(axr_ctx* ctx, FrameArena& fa) {
-    int r = arena_read_counters(ctx->err, fa);
-    if (r || !fa.host_counters[CNT_ERROR]) return r;
+    if (!fa.host_counters[CNT_ERROR]) return AXR_OK;
     FrameSize fs;
     fs.faces = fa.rec.bytes / sizeof(TriRec);
     fs.tiles = fa.tile_count.bytes / 4 - 1;
@@ -693,11 +716,23 @@ static int arena_settle_overflow(axr_ctx* ctx, FrameArena& fa) {
 int axr_sync(axr_ctx* ctx) {
     if (!ctx) return AXR_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
-    int r = ctx_sync_all(ctx);
-    if (r) return r;
-    r = arena_settle_overflow(ctx, ctx->main);
-    for (auto& a : ctx->alt) {
-        const int r2 = a.stream ? arena_settle_overflow(ctx, a) : AXR_OK;
+    // an event behind each arena's queued work, all recorded before the
+    // first wait: the streams' completions then overlap (a stream wait per
+    // arena in turn costs ~10 us each); the counters come back only from an
+    // arena whose kernels flagged an error
+    int live[axr_ctx::kArenas];
+    int n = 0;
+    for (int k = 0; k < axr_ctx::kArenas; ++k)
+        if (arena_no(ctx, k).stream) live[n++] = k;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n && e == hipSuccess; ++i) e = hipEventRecord(ctx->sync_ev[live[i]], arena_no(ctx, live[i]).stream);
+    for (int i = 0; i < n && e == hipSuccess; ++i) e = hipEventSynchronize(ctx->sync_ev[live[i]]);
+    if (e != hipSuccess) return hip_fail(ctx->err, e, "hipEventSynchronize");
+    int r = AXR_OK;
+    for (int i = 0; i < n; ++i) {
+        FrameArena& fa = arena_no(ctx, live[i]);
+        int r2 = arena_counters_after_wait(ctx->err, fa);
+        if (!r2) r2 = arena_settle_overflow(ctx, fa);
         if (!r) r = r2;
     }
     const int r3 = noalloc_settle(ctx);
@@ -1051,7 +1086,9 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
         if ((r = ctx_launch(ctx, ka))) return r;
         fb->clear_pending = false;   // written by this launch, even by a frame that overflows
         if (async) return AXR_OK;
-        if ((r = arena_read_counters(ctx->err, ctx->main))) return r;
+        const hipError_t e = hipStreamSynchronize(ctx->main.stream);
+        if (e != hipSuccess) return hip_fail(ctx->err, e, "hipStreamSynchronize");
+        if ((r = arena_counters_after_wait(ctx->err, ctx->main))) return r;
         if (!ctx->main.host_counters[CNT_ERROR]) return AXR_OK;
         r = arena_check_overflow(ctx, ctx->err, ctx->main, fs);
         if (r != AXR_ERR_OVERFLOW) return r;

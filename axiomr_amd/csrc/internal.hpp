@@ -70,7 +70,8 @@ struct FrameArena {
     Arena a{};
     uint64_t gen = 0;          // bumped whenever a buffer moves: captured graphs are stale
     bool dirty = true;         // bin counts / counters need a k_prep before the next frame
-    uint32_t* host_counters = nullptr;   // pinned, CNT_COUNT words
+    uint32_t* host_counters = nullptr;   // pinned, CNT_COUNT words (then host_err, on its own cache line)
+    uint32_t* host_err = nullptr;        // pinned word the kernels set nonzero with any error bit
     uint64_t* sig = nullptr;   // signal memory its stream writes after a framebuffer pass (null: use events)
 };
 
@@ -95,6 +96,7 @@ struct axr_ctx {
     static constexpr int kArenas = 4;
     axr::FrameArena alt[kArenas - 1];
     hipEvent_t join_ev[kArenas] = {};   // join_ev[k]: recorded on arena k's stream for another to wait on
+    hipEvent_t sync_ev[kArenas] = {};   // sync_ev[k]: recorded on arena k's stream for axr_sync to wait on
     int next_arena = 0;                 // arena of the next asynchronous visibility-mode draw
     int last_arena = 0;                 // arena whose stream took the latest framebuffer work
     bool alt_unjoined[kArenas - 1] = {};  // alt[k] took work main has not waited for
@@ -183,10 +185,14 @@ void arena_destroy(axr_ctx* ctx, FrameArena& fa);
 uint64_t arena_bytes(const FrameArena& fa);
 int arena_ensure(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs, const ArenaNeeds& need = {});
 int arena_read_counters(ErrSink& err, FrameArena& fa);
+// With the arena's work finished: the counters copied back if a kernel
+// flagged an error in the pinned error word, else their host copy's error
+// word zeroed.
+int arena_counters_after_wait(ErrSink& err, FrameArena& fa);
 // k_prep if the arena is fresh or dirty, then the frame's kernels; ev (may
 // be null) times them.
 int arena_launch(ErrSink& err, FrameArena& fa, const KArgs& ka, StageEvents* ev, const FbWait& wait = {});
-// After arena_read_counters: AXR_OK, or grow the arena to the reported
+// After arena_counters_after_wait (or arena_read_counters): AXR_OK, or grow the arena to the reported
 // high-water marks, reset it and return AXR_ERR_OVERFLOW.
 int arena_check_overflow(axr_ctx* ctx, ErrSink& err, FrameArena& fa, const FrameSize& fs);
 

@@ -141,8 +141,17 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + (uint32_t)__popcll(mask & __lanemask_lt());
 }
 
-__device__ __forceinline__ void set_error(const Arena& ar, uint32_t bit, uint32_t need_slot, uint32_t need) {
+// A frame's sticky error bit.  The pinned host word is set as well (a
+// system-scope store, seen by the host once the kernel has completed), so
+// the host learns of an error without copying the counters back after every
+// wait; errors are rare, so the store over the host link is too.
+__device__ __forceinline__ void raise_error(const Arena& ar, uint32_t bit) {
     atomicOr(&ar.counters[CNT_ERROR], bit);
+    if (ar.host_err) __hip_atomic_store(ar.host_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ void set_error(const Arena& ar, uint32_t bit, uint32_t need_slot, uint32_t need) {
+    raise_error(ar, bit);
     atomicMax(&ar.counters[need_slot], need);
 }
 
@@ -444,7 +453,7 @@ __device__ __forceinline__ uint32_t big_chunks(uint32_t area) { return (area + B
 
 __device__ __forceinline__ void big_store(const Arena& ar, const TriRec& r, uint32_t ri, uint32_t i0, uint32_t nch) {
     if (ri >= ar.cap_big_recs || i0 + nch > ar.cap_big_items || i0 + nch < i0) {
-        atomicOr(&ar.counters[CNT_ERROR], ERR_BIG);
+        raise_error(ar, ERR_BIG);
         atomicMax(&ar.counters[CNT_NEED_BIG_RECS], ri + 1);
         atomicMax(&ar.counters[CNT_NEED_BIG_ITEMS], i0 + nch);
         return;
@@ -512,7 +521,7 @@ __device__ __forceinline__ void setup_face(const KArgs& ka, const Arena& ar, uin
     if (want_clip) {
         if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g;
         else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
-        if (ka.skip_clip) atomicOr(&ar.counters[CNT_ERROR], ERR_SKIPPED);
+        if (ka.skip_clip) raise_error(ar, ERR_SKIPPED);
     }
     emit_pairs_wave<T>(ar, V, binnable, rc, g, g >> 6);
 }
@@ -620,7 +629,7 @@ __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
             if (cslot < ar.cap_clip_faces) ar.clip_faces[cslot] = g[k];
             else set_error(ar, ERR_CLIP_FACES, CNT_NEED_CLIP_FACES, cslot + 1);
         }
-        if ((want_clip && ka.skip_clip) || (big && ka.skip_big)) atomicOr(&ar.counters[CNT_ERROR], ERR_SKIPPED);
+        if ((want_clip && ka.skip_clip) || (big && ka.skip_big)) raise_error(ar, ERR_SKIPPED);
         big_append_wave(ar, big, r, area);
     }
 }

@@ -198,11 +198,13 @@ public:
         hipError_t e = hipEventQuery(L.done);
         if (e == hipErrorNotReady) return 0;
         if (e != hipSuccess) return hip_fail(err_, e, "server batch");
+        int r = arena_counters_after_wait(err_, L.fa);
+        if (r) return r;
         if (!L.fa.host_counters[CNT_ERROR]) return 1;
         // arena overflow: the raster stage did not run, so the framebuffers
         // hold (at most) their clear; grow to the reported needs and redraw
         if (++L.attempts > 8) return fail(err_, AXR_ERR_OVERFLOW, "server batch kept overflowing its arena");
-        int r = arena_check_overflow(ctx_, err_, L.fa, L.fs);
+        r = arena_check_overflow(ctx_, err_, L.fa, L.fs);
         if (r != AXR_ERR_OVERFLOW) return r < 0 ? r : AXR_ERR_OVERFLOW;
         ++overflow_retries_;
         r = enqueue(L);
@@ -216,9 +218,8 @@ private:
     int enqueue(Lane& L) {
         int r = arena_launch(err_, L.fa, L.ka, nullptr);
         if (r) return r;
-        hipError_t e = hipMemcpyAsync(L.fa.host_counters, L.fa.counters.p, CNT_COUNT * 4, hipMemcpyDeviceToHost,
-                                      L.fa.stream);
-        if (e == hipSuccess) e = hipEventRecord(L.done, L.fa.stream);
+        // the counters come back only if the batch flags an error (query)
+        const hipError_t e = hipEventRecord(L.done, L.fa.stream);
         return e == hipSuccess ? AXR_OK : hip_fail(err_, e, "server batch enqueue");
     }
 
