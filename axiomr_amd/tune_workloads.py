@@ -1,9 +1,12 @@
 """Tune the binning pipeline for the benchmark workloads on this GPU and
 store the result in axiomr_amd/tuning/mi355x.json:
 
-    python -m axiomr_amd.tune_workloads [plain] [batch] [serve]
+    python -m axiomr_amd.tune_workloads [plain] [batch] [serve] [uv_sphere_800x600] [soup_4M_1080p]
 
 plain: one 1920x1080 frame of the 1,003,520-triangle icosphere;
+uv_sphere_800x600: one frame of the 20,000-triangle UV sphere (Gouraud x
+checker, the reference application's scene);
+soup_4M_1080p: one 1920x1080 frame of the 4M-triangle overdraw soup;
 batch: the 256-view / 32-mesh batch in one graph launch;
 serve: a burst of 400 requests of the served workload (64 meshes, mixed
 resolutions) through a 3-lane render server.
@@ -24,7 +27,12 @@ def log(m):
 
 
 def tune_plain(ctx):
-    scene = S.icosphere_scene(n=224)
+    return tune_scene(ctx, S.icosphere_scene(n=224), "plain", 200)
+
+
+def tune_scene(ctx, scene, workload, inner):
+    """Asynchronous clear + draw frames of one scene into one framebuffer,
+    `inner` of them per timed run."""
     gs = RS.GpuScene(ctx, scene)
     fb = Framebuffer(ctx, scene.width, scene.height)
     p = RS.params_of(scene, flags=1)
@@ -36,10 +44,10 @@ def tune_plain(ctx):
             L.axr_draw(ctx.handle, fb.handle, gs.mesh.handle, C.byref(p))
         return frame
 
-    cached = AT.load("plain")
+    cached = AT.load(workload) or AT.load("plain")
     if cached:
         ctx.set_config(**cached["config"])
-    res = AT.tune(ctx, make_frame, log, reps=5, inner=200)
+    res = AT.tune(ctx, make_frame, log, reps=5, inner=inner)
     fb.close()
     gs.close()
     return res
@@ -80,10 +88,13 @@ def tune_serve(ctx):
 
 
 def main(argv):
-    which = argv or ["plain", "batch", "serve"]
+    which = argv or ["plain", "batch", "serve", "uv_sphere_800x600", "soup_4M_1080p"]
+    runs = {"plain": tune_plain, "batch": tune_batch, "serve": tune_serve,
+            "uv_sphere_800x600": lambda c: tune_scene(c, S.uv_sphere_scene(), "uv_sphere_800x600", 200),
+            "soup_4M_1080p": lambda c: tune_scene(c, S.soup_scene(), "soup_4M_1080p", 10)}
     ctx = Context(0)
     for w in which:
-        res = {"plain": tune_plain, "batch": tune_batch, "serve": tune_serve}[w](ctx)
+        res = runs[w](ctx)
         log(f"[autotune] {w}: best {res['best']} {res['seconds'] * 1e3:.3f} ms")
         AT.store(w, res)
         print(f"{w} {res['best']} {res['seconds'] * 1e3:.4f} ms", flush=True)
