@@ -306,7 +306,7 @@ def batch_kernel_bytes(wl, covered, ntex=2):
             "k_clear_views": 8 * wl.total_pixels, "k_vary": 0}
 
 
-SERVE_RATES = (100, 250, 500, 1000, 2500, 5000)
+SERVE_RATES = (100, 250, 500, 1000, 2500, 5000, 10000, 20000, 40000)
 
 
 def run_serve(wl, rates, seconds_per_rate, lanes, budget, tile, use_tuned, burst=2000):
