@@ -132,3 +132,34 @@ def test_replay_open_loop(ctx, served):
     assert (lat >= 0).all() and np.isfinite(lat).all()
     assert st["completed"] == 300 and st["fb_in_use_bytes"] == 0
     assert float(np.percentile(lat, 50)) < 0.5
+
+
+def test_lane_overflow_is_redrawn():
+    """A lane sized for small batches (max_batch_faces 4096) takes a request
+    with far more large and clipped triangles than its arena holds: the
+    batch's kernels flag the overflow, the lane grows its arena to the
+    reported needs and draws the request again, and its pixels then match
+    the checker's."""
+    from axiomr_amd import _native as N
+    from axiomr_amd import render as R
+    sc = S.soup_scene(ntri=60_000, width=480, height=270, seed=9)
+    ref = H.render_oracle(sc)
+    c = R.Context(0)
+    try:
+        c.set_config(tile_size=16, raster_mode=N.RASTER_VISIBILITY, small_footprint=12)
+        gs = H.GpuScene(c, sc)
+        try:
+            with Server(c, lanes=1, max_batch_views=4, max_batch_faces=4096, with_ids=True) as srv:
+                t = srv.submit(gs.mesh, sc.width, sc.height, H.params_of(sc))
+                srv.wait(t, 60.0)
+                assert srv.poll(t) == DONE
+                got = srv.read(t)
+                st = srv.stats()
+                srv.release(t)
+        finally:
+            gs.close()
+    finally:
+        c.close()
+    assert st["overflow_retries"] >= 1, st
+    assert st["failed"] == 0 and st["completed"] == 1, st
+    H.assert_parity(H.compare(got, ref), "served soup after an arena overflow")
