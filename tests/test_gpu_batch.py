@@ -49,3 +49,25 @@ def test_batch_views_match_checker(ctx):
             H.assert_parity(rep, f"batch view {i}")
     finally:
         gb.close()
+
+
+def test_batch_rejects_a_framebuffer_in_two_views(ctx):
+    """Every view of a batch resolves at the same time, so two views drawing
+    into one framebuffer would race on its pixels: creation refuses it with
+    AXR_ERR_INVALID (and accepts the same views on two framebuffers)."""
+    from axiomr_amd import render as R
+    from axiomr_amd._native import AXR_ERR_INVALID, AxrError
+    sc = S.icosphere_scene(n=8, width=64, height=48, tex=16)
+    gs = H.GpuScene(ctx, sc)
+    fbs = [R.Framebuffer(ctx, sc.width, sc.height) for _ in range(2)]
+    p = H.params_of(sc)
+    try:
+        with pytest.raises(AxrError) as ei:
+            R.Batch(ctx, [fbs[0], fbs[0]], [gs.mesh, gs.mesh], [p, p])
+        assert ei.value.code == AXR_ERR_INVALID
+        ok = R.Batch(ctx, fbs, [gs.mesh, gs.mesh], [p, p])
+        ok.close()
+    finally:
+        for fb in fbs:
+            fb.close()
+        gs.close()
