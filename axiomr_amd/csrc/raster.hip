@@ -1162,8 +1162,8 @@ struct WalkPieces {
 
 // All 64 lanes call it with their own triangle (w = h = 0 for none).
 template <int T>
-__device__ __forceinline__ void walk_wave(WalkPieces* wp, const RasterTri& rt, int x0, int y0, int w, int h,
-                                          uint32_t key, unsigned long long* keys, int ox, int oy) {
+__device__ __forceinline__ void walk_wave(WalkPieces* wp, const Arena&, const RasterTri& rt, uint32_t, int x0, int y0,
+                                          int w, int h, uint32_t key, unsigned long long* keys, int ox, int oy) {
     const int fp = w * h;
     const float inv_w = w ? 1.0f / (float)w : 0.0f;
     walk_span<T>(rt, x0 + ox, y0 + oy, w, 0, imin(fp, WALK_CAP), key, keys, ox, oy, inv_w);
@@ -1207,6 +1207,59 @@ __device__ __forceinline__ void walk_wave(WalkPieces* wp, const RasterTri& rt, i
     }
 }
 
+// The same for a kernel that needs its LDS small (a raster kernel whose
+// occupancy is bounded by both its registers and its LDS): per lane only the
+// bin entry's record reference and the rectangle; the lane that walks a piece
+// loads the record again and repeats its edge setup (the same operations on
+// the same bits).  Three words per lane instead of seventeen.
+struct WalkRefs {
+    uint32_t ref[64];
+    uint32_t rect[64];    // x0 | y0 << 8 | w << 16 | h << 24 (bin coordinates)
+    uint32_t start[64];
+};
+
+template <int T>
+__device__ __forceinline__ void walk_wave(WalkRefs* wp, const Arena& ar, const RasterTri& rt, uint32_t ref, int x0,
+                                          int y0, int w, int h, uint32_t key, unsigned long long* keys, int ox,
+                                          int oy) {
+    const int fp = w * h;
+    const float inv_w = w ? 1.0f / (float)w : 0.0f;
+    walk_span<T>(rt, x0 + ox, y0 + oy, w, 0, imin(fp, WALK_CAP), key, keys, ox, oy, inv_w);
+    const uint32_t pieces = fp > WALK_CAP ? (uint32_t)(fp - 1) / WALK_CAP : 0u;
+    const uint32_t incl = wave_incl_scan(pieces);
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0) return;
+    const uint32_t lane = __lane_id();
+    wp->start[lane] = incl - pieces;
+    if (pieces) {
+        wp->ref[lane] = ref;
+        wp->rect[lane] = (uint32_t)x0 | ((uint32_t)y0 << 8) | ((uint32_t)w << 16) | ((uint32_t)h << 24);
+    }
+    // the wave's LDS writes before its reads of other lanes' entries
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t j = lane; j < total; j += 64u) {
+        uint32_t o = 0;
+        for (uint32_t step = 32; step; step >>= 1)
+            if (wp->start[o + step] <= j) o += step;
+        const TriRec r = rec_of(ar, wp->ref[o]);
+        RasterTri t;
+        raster_setup(r, t);
+        const uint32_t rect = wp->rect[o];
+        const int rx = (int)(rect & 255u), ry = (int)((rect >> 8) & 255u);
+        const int rw = (int)((rect >> 16) & 255u), rh = (int)(rect >> 24);
+        const int q0 = WALK_CAP * (int)(j - wp->start[o] + 1);
+        walk_span<T>(t, rx + ox, ry + oy, rw, q0, imin(q0 + WALK_CAP, rw * rh), r.c.y, keys, ox, oy, 1.0f / (float)rw);
+    }
+}
+
+// The full layout keeps the walkers' loads in LDS; the normal-mapped
+// point-light raster kernel takes the compact one, which with its launch
+// bound (raster_min_waves) lets five waves share a SIMD instead of four.
+template <int SK> struct PiecesOf { using type = WalkPieces; };
+template <> struct PiecesOf<SHADER_PHONG_POINT> { using type = WalkRefs; };
+
 template <bool BATCH>
 __device__ __forceinline__ const ViewDesc& bin_view(const KArgs& ka, const ViewDesc* sview) {
     if constexpr (BATCH) return *sview;
@@ -1218,7 +1271,7 @@ __device__ __forceinline__ const ViewDesc& bin_view(const KArgs& ka, const ViewD
 template <int T, int NT, bool BATCH, int SK>
 __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uint32_t t, unsigned long long* keys,
                                            uint32_t* bigq, uint32_t& nbig, uint16_t* plist, ViewDesc* sview,
-                                           WalkPieces* pieces) {
+                                           typename PiecesOf<SK>::type* pieces) {
     if constexpr (BATCH) {
         const uint32_t vi = ka.tile_views[t];
         const uint32_t* src = reinterpret_cast<const uint32_t*>(ka.table + vi);
@@ -1237,14 +1290,14 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
     // every wave has read n: the count is consumed, leave it zero for the next frame
     if (threadIdx.x == 0) ar.tile_count[t] = 0;
 
-    WalkPieces* wp = pieces + (threadIdx.x >> 6);
+    typename PiecesOf<SK>::type* wp = pieces + (threadIdx.x >> 6);
     for (uint32_t b0 = 0; b0 < n; b0 += NT) {
         const uint32_t i = b0 + threadIdx.x;
         RasterTri rt;
         int x0 = 0, y0 = 0, w = 0, h = 0;
-        uint32_t key = 0;
+        uint32_t key = 0, ref = 0;
         if (i < n) {
-            const uint32_t ref = ar.bins[off + i];
+            ref = ar.bins[off + i];
             const TriRec r = rec_of(ar, ref);
             if (raster_setup(r, rt)) {
                 x0 = imax(ox, rt.fx0);
@@ -1260,7 +1313,7 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
                 key = r.c.y;
             }
         }
-        walk_wave<T>(wp, rt, x0 - ox, y0 - oy, w, h, key, keys, ox, oy);
+        walk_wave<T>(wp, ar, rt, ref, x0 - ox, y0 - oy, w, h, key, keys, ox, oy);
         __syncthreads();
         const uint32_t nb = nbig;
         for (uint32_t q = 0; q < nb; ++q) {
@@ -1298,12 +1351,19 @@ __device__ __forceinline__ void raster_bin(const KArgs& ka, const Arena& ar, uin
 // (see k_resolve).
 constexpr uint32_t NUM_XCD = 8;
 
-// Six waves per SIMD (at most 80 VGPRs) for the shader kinds whose raster
-// kernel fits that without spilling; the normal-mapped shaders take the
-// registers they need (held to 80, the point-light one spilled 22 VGPRs and
-// ran 1.68 ms on the 256-view batch; unconstrained, 1.38 ms).
+// Register bounds per shader kind (waves per SIMD the registers allow; the
+// LDS of a 128-thread block at 32 px bins, 19.9 KB with the full walk
+// pieces, allows four).  At most 80 VGPRs for the kinds that fit without
+// spilling.  The point-light kind at five waves (at most 96 VGPRs, 8
+// spilled) with the compact walk pieces (12.7 KB of LDS): the 256-view
+// batch's k_raster 1.224 -> 1.192 ms; the compact pieces alone cost
+// 0.04 ms and slowed the flat-colour soup by 11 %, so the other kinds keep
+// the full ones.  Phong and PBR take the registers they need (held to 80,
+// the point-light kind had spilled 22 VGPRs and run 1.68 against 1.38 ms).
 constexpr int raster_min_waves(int sk) {
-    return (sk == SHADER_FLAT || sk == SHADER_GOURAUD || sk == SHADER_FLAT_COLOR) ? 6 : 1;
+    return (sk == SHADER_FLAT || sk == SHADER_GOURAUD || sk == SHADER_FLAT_COLOR) ? 6
+           : sk == SHADER_PHONG_POINT                                            ? 5
+                                                                                 : 1;
 }
 
 template <int T, int NT, bool BATCH, int SK>
@@ -1313,7 +1373,7 @@ __global__ __launch_bounds__(NT, raster_min_waves(SK)) void k_raster(KArgs ka, A
     __shared__ uint16_t plist[T * T];
     __shared__ uint32_t nbig;
     __shared__ ViewDesc sview[1];
-    __shared__ WalkPieces pieces[NT / 64];
+    __shared__ typename PiecesOf<SK>::type pieces[NT / 64];
     if (ar.counters[CNT_ERROR]) return;
     // the pair overflow count was last needed by the scatter
     if (blockIdx.x == 0 && threadIdx.x == 0) ar.counters[CNT_OVF_PAIRS] = 0;
