@@ -121,3 +121,22 @@ def test_two_rounds_of_descent(monkeypatch):
     keys = [tuple(sorted((k, v) for k, v in t.items() if k != "seconds")) for t in res["trials"]]
     assert len(keys) == len(set(keys))
     assert ctx.config() == best
+
+
+def test_capacity_takes_the_smallest_within_slack(monkeypatch):
+    """The bin capacity sizes the pair regions, so a larger one is kept only
+    when it is more than 1 % faster: here 1024 is 0.5 % faster than 256
+    (kept: 256) and 256 is 2 % faster than 128 (so not 128)."""
+    cap_cost = {64: 10.5, 128: 10.2, 256: 10.0, 512: 10.0, 1024: 9.95}
+
+    def cost(c):
+        t = cap_cost[c["bin_capacity"]] if c["raster_mode"] == 0 else 20.0
+        return t + (0.0 if (c["tile_size"], c["raster_threads"], c["small_footprint"]) == (32, 128, 256) else 1.0)
+
+    ctx = _TuneCtx({"tile_size": 32, "raster_threads": 128, "bin_capacity": 128, "small_footprint": 256,
+                    "raster_mode": 0})
+    monkeypatch.setattr(AT, "time_config", lambda c, frame, reps, inner: cost(c.config()))
+    res = AT.tune(ctx, lambda: (lambda: None))
+    assert res["best"]["bin_capacity"] == 256, res["best"]
+    assert res["seconds"] == pytest.approx(10.0)
+    assert ctx.config() == res["best"]
