@@ -241,9 +241,10 @@ struct KArgs {
     const uint32_t* vary_firsts = nullptr;
     uint32_t vary_njobs = 0;
     uint32_t vary_total = 0;
-    int32_t skip_clip = 0;      // one view: no face can cross a frustum plane, so the
-    int32_t skip_big = 0;       // clipper (and, when no triangle can exceed the
-                                // footprint, the large-triangle pass) is not launched
+    int32_t skip_clip = 0;      // one view: the mesh's bounding sphere lies inside every
+    int32_t skip_big = 0;       // frustum plane, so no face is classified and the
+                                // clipper is not launched (nor, when no triangle can
+                                // exceed the footprint, the large-triangle pass)
     int32_t depth_is_clear = 0; // binned mode: every framebuffer's depth plane holds
                                 // clear_depth (written just before, by k_clear /
                                 // k_clear_views): winners are tested against that

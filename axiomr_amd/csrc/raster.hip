@@ -603,7 +603,10 @@ __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
             const vec4 c0 = V.mvp * make4(p0[k], 1.0f);
             const vec4 c1 = V.mvp * make4(p1[k], 1.0f);
             const vec4 c2 = V.mvp * make4(p2[k], 1.0f);
-            const int cls = classify(c0, c1, c2);
+            // the host's bound put the mesh's bounding sphere inside every
+            // frustum plane (skip_clip): every face classifies as inside
+            int cls = CLS_INSIDE;
+            if (!ka.skip_clip) cls = classify(c0, c1, c2);
             want_clip = cls == CLS_CLIP;
             if (cls == CLS_INSIDE) {
                 r = make_rec(c0, c1, c2, V.W, V.H);
@@ -1436,7 +1439,7 @@ __global__ __launch_bounds__(256) void k_big(KArgs ka, Arena ar) {
 // absorbs): returns whether it did, with its depth and packed colour.
 template <class S>
 __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
-                                                int py, float dfb, float& z, uint32_t& bgra) {
+                                                int py, float dfb, float& z, uint32_t& bgra, bool no_clip = false) {
     const uint32_t key = (uint32_t)k;
     const uint32_t s = key >> 3, j = key & 7u;
     const DevGroup G = group_of(V, s);
@@ -1464,7 +1467,8 @@ __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& 
     TriRec r;
     bool clipped = false;
     uint32_t slot = 0;
-    if (classify(c0, c1, c2) == CLS_CLIP) {
+    // no_clip: the draw's faces all lie inside the frustum (the host's bound)
+    if (!no_clip && classify(c0, c1, c2) == CLS_CLIP) {
         clipped = true;
         slot = ar.clip_base[V.face_base + s] + j;   // written by k_clip
         r = ar.clip_rec[slot];
@@ -1679,7 +1683,7 @@ __global__ __launch_bounds__(256) void k_shade(KArgs ka, Arena ar) {
         if (k == EMPTY_KEY) continue;
         float z;
         uint32_t bgra;
-        const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra);
+        const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra, ka.skip_clip != 0);
         vk[pix] = drawn ? ((unsigned long long)__float_as_uint(z) << 32) | (IDS ? (uint32_t)k : bgra) : EMPTY_KEY;
         if (IDS && drawn) ar.vcol[V.pix_base + pix] = bgra;
     }
