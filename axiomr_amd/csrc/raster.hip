@@ -1153,19 +1153,61 @@ __device__ __forceinline__ void walk_span(const RasterTri& rt, int x0, int y0, i
 }
 
 // A wave's triangles whose rectangles outlast WALK_CAP pixels, for the
-// pieces: edge setup, rectangle and key per lane, and the first piece of
-// each lane in the wave's numbering.
+// pieces: per lane what a walker needs of its triangle, and the first piece
+// of each lane in the wave's numbering.  Two layouts:
+//   WalkPieces  the edge setup itself, rectangle and key (17 words a lane);
+//   WalkRefs    the bin entry's record reference and the rectangle (3 words):
+//               the walker loads the record again and repeats the edge setup
+//               (the same operations on the same bits), for a raster kernel
+//               whose occupancy its LDS would otherwise bound.
 struct WalkPieces {
     float4 e0[64];        // a0, b0, c0, a1
     float4 e1[64];        // b1, c1, a2, b2
     float4 e2[64];        // c2, inv_area, z0, z1
-    float4 e3[64];        // z2, bits(fx0), bits(x0 | y0 << 8 | w << 16 | h << 24) (bin coordinates), bits(key)
+    float4 e3[64];        // z2, bits(fx0), bits(rect), bits(key)
     uint32_t start[64];
 };
+struct WalkRefs {
+    uint32_t ref[64];
+    uint32_t rect[64];
+    uint32_t start[64];
+};
+// rect: x0 | y0 << 8 | w << 16 | h << 24, in bin coordinates
 
-// All 64 lanes call it with their own triangle (w = h = 0 for none).
-template <int T>
-__device__ __forceinline__ void walk_wave(WalkPieces* wp, const Arena&, const RasterTri& rt, uint32_t, int x0, int y0,
+__device__ __forceinline__ void put_piece(WalkPieces* wp, uint32_t lane, const RasterTri& rt, uint32_t, uint32_t rect,
+                                          uint32_t key) {
+    wp->e0[lane] = make_float4(rt.a0, rt.b0, rt.c0, rt.a1);
+    wp->e1[lane] = make_float4(rt.b1, rt.c1, rt.a2, rt.b2);
+    wp->e2[lane] = make_float4(rt.c2, rt.inv_area, rt.z0, rt.z1);
+    wp->e3[lane] = make_float4(rt.z2, __int_as_float(rt.fx0), __uint_as_float(rect), __uint_as_float(key));
+}
+__device__ __forceinline__ void get_piece(const WalkPieces* wp, const Arena&, uint32_t o, RasterTri& t, uint32_t& rect,
+                                          uint32_t& key) {
+    const float4 f0 = wp->e0[o], f1 = wp->e1[o], f2 = wp->e2[o], f3 = wp->e3[o];
+    t.a0 = f0.x; t.b0 = f0.y; t.c0 = f0.z; t.a1 = f0.w;
+    t.b1 = f1.x; t.c1 = f1.y; t.a2 = f1.z; t.b2 = f1.w;
+    t.c2 = f2.x; t.inv_area = f2.y; t.z0 = f2.z; t.z1 = f2.w;
+    t.z2 = f3.x; t.fx0 = __float_as_int(f3.y);
+    rect = __float_as_uint(f3.z);
+    key = __float_as_uint(f3.w);
+}
+__device__ __forceinline__ void put_piece(WalkRefs* wp, uint32_t lane, const RasterTri&, uint32_t ref, uint32_t rect,
+                                          uint32_t) {
+    wp->ref[lane] = ref;
+    wp->rect[lane] = rect;
+}
+__device__ __forceinline__ void get_piece(const WalkRefs* wp, const Arena& ar, uint32_t o, RasterTri& t, uint32_t& rect,
+                                          uint32_t& key) {
+    const TriRec r = rec_of(ar, wp->ref[o]);
+    raster_setup(r, t);
+    rect = wp->rect[o];
+    key = r.c.y;
+}
+
+// All 64 lanes call it with their own triangle (w = h = 0 for none); ref is
+// its bin entry's record reference.
+template <int T, class P>
+__device__ __forceinline__ void walk_wave(P* wp, const Arena& ar, const RasterTri& rt, uint32_t ref, int x0, int y0,
                                           int w, int h, uint32_t key, unsigned long long* keys, int ox, int oy) {
     const int fp = w * h;
     const float inv_w = w ? 1.0f / (float)w : 0.0f;
@@ -1176,15 +1218,9 @@ __device__ __forceinline__ void walk_wave(WalkPieces* wp, const Arena&, const Ra
     if (total == 0) return;
     const uint32_t lane = __lane_id();
     wp->start[lane] = incl - pieces;
-    if (pieces) {
-        wp->e0[lane] = make_float4(rt.a0, rt.b0, rt.c0, rt.a1);
-        wp->e1[lane] = make_float4(rt.b1, rt.c1, rt.a2, rt.b2);
-        wp->e2[lane] = make_float4(rt.c2, rt.inv_area, rt.z0, rt.z1);
-        wp->e3[lane] = make_float4(rt.z2, __int_as_float(rt.fx0),
-                                   __uint_as_float((uint32_t)x0 | ((uint32_t)y0 << 8) | ((uint32_t)w << 16) |
-                                                   ((uint32_t)h << 24)),
-                                   __uint_as_float(key));
-    }
+    if (pieces)
+        put_piece(wp, lane, rt, ref, (uint32_t)x0 | ((uint32_t)y0 << 8) | ((uint32_t)w << 16) | ((uint32_t)h << 24),
+                  key);
     // the wave's LDS writes before its reads of other lanes' entries
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1195,71 +1231,19 @@ __device__ __forceinline__ void walk_wave(WalkPieces* wp, const Arena&, const Ra
         uint32_t o = 0;
         for (uint32_t step = 32; step; step >>= 1)
             if (wp->start[o + step] <= j) o += step;
-        const float4 f0 = wp->e0[o], f1 = wp->e1[o], f2 = wp->e2[o], f3 = wp->e3[o];
         RasterTri t;
-        t.a0 = f0.x; t.b0 = f0.y; t.c0 = f0.z; t.a1 = f0.w;
-        t.b1 = f1.x; t.c1 = f1.y; t.a2 = f1.z; t.b2 = f1.w;
-        t.c2 = f2.x; t.inv_area = f2.y; t.z0 = f2.z; t.z1 = f2.w;
-        t.z2 = f3.x; t.fx0 = __float_as_int(f3.y);
-        const uint32_t rect = __float_as_uint(f3.z);
+        uint32_t rect, pkey;
+        get_piece(wp, ar, o, t, rect, pkey);
         const int rx = (int)(rect & 255u), ry = (int)((rect >> 8) & 255u);
         const int rw = (int)((rect >> 16) & 255u), rh = (int)(rect >> 24);
         const int q0 = WALK_CAP * (int)(j - wp->start[o] + 1);
-        walk_span<T>(t, rx + ox, ry + oy, rw, q0, imin(q0 + WALK_CAP, rw * rh), __float_as_uint(f3.w), keys, ox, oy,
-                     1.0f / (float)rw);
+        walk_span<T>(t, rx + ox, ry + oy, rw, q0, imin(q0 + WALK_CAP, rw * rh), pkey, keys, ox, oy, 1.0f / (float)rw);
     }
 }
 
-// The same for a kernel that needs its LDS small (a raster kernel whose
-// occupancy is bounded by both its registers and its LDS): per lane only the
-// bin entry's record reference and the rectangle; the lane that walks a piece
-// loads the record again and repeats its edge setup (the same operations on
-// the same bits).  Three words per lane instead of seventeen.
-struct WalkRefs {
-    uint32_t ref[64];
-    uint32_t rect[64];    // x0 | y0 << 8 | w << 16 | h << 24 (bin coordinates)
-    uint32_t start[64];
-};
-
-template <int T>
-__device__ __forceinline__ void walk_wave(WalkRefs* wp, const Arena& ar, const RasterTri& rt, uint32_t ref, int x0,
-                                          int y0, int w, int h, uint32_t key, unsigned long long* keys, int ox,
-                                          int oy) {
-    const int fp = w * h;
-    const float inv_w = w ? 1.0f / (float)w : 0.0f;
-    walk_span<T>(rt, x0 + ox, y0 + oy, w, 0, imin(fp, WALK_CAP), key, keys, ox, oy, inv_w);
-    const uint32_t pieces = fp > WALK_CAP ? (uint32_t)(fp - 1) / WALK_CAP : 0u;
-    const uint32_t incl = wave_incl_scan(pieces);
-    const uint32_t total = __shfl(incl, 63);
-    if (total == 0) return;
-    const uint32_t lane = __lane_id();
-    wp->start[lane] = incl - pieces;
-    if (pieces) {
-        wp->ref[lane] = ref;
-        wp->rect[lane] = (uint32_t)x0 | ((uint32_t)y0 << 8) | ((uint32_t)w << 16) | ((uint32_t)h << 24);
-    }
-    // the wave's LDS writes before its reads of other lanes' entries
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t j = lane; j < total; j += 64u) {
-        uint32_t o = 0;
-        for (uint32_t step = 32; step; step >>= 1)
-            if (wp->start[o + step] <= j) o += step;
-        const TriRec r = rec_of(ar, wp->ref[o]);
-        RasterTri t;
-        raster_setup(r, t);
-        const uint32_t rect = wp->rect[o];
-        const int rx = (int)(rect & 255u), ry = (int)((rect >> 8) & 255u);
-        const int rw = (int)((rect >> 16) & 255u), rh = (int)(rect >> 24);
-        const int q0 = WALK_CAP * (int)(j - wp->start[o] + 1);
-        walk_span<T>(t, rx + ox, ry + oy, rw, q0, imin(q0 + WALK_CAP, rw * rh), r.c.y, keys, ox, oy, 1.0f / (float)rw);
-    }
-}
-
-// The full layout keeps the walkers' loads in LDS; the normal-mapped
-// point-light raster kernel takes the compact one, which with its launch
-// bound (raster_min_waves) lets five waves share a SIMD instead of four.
+// The point-light raster kernel takes the compact layout, which with its
+// launch bound (raster_min_waves) lets five waves share a SIMD instead of
+// four; the other kinds keep the walkers' loads in LDS.
 template <int SK> struct PiecesOf { using type = WalkPieces; };
 template <> struct PiecesOf<SHADER_PHONG_POINT> { using type = WalkRefs; };
 
