@@ -559,7 +559,11 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
 // draws, no gain at 1M faces).
 constexpr uint32_t VIS_FACES_PER_LANE = 1;
 
-template <int T, bool BATCH>
+// INSIDE: the host's bound put the mesh's bounding sphere inside every
+// frustum plane (ka.skip_clip), so every face classifies as inside and the
+// classification is left out (a separate instantiation: a runtime test of the
+// flag is compiled to a select, which still computes it).
+template <int T, bool BATCH, bool INSIDE = false>
 __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
     constexpr uint32_t F = VIS_FACES_PER_LANE;
     if (ar.counters[CNT_ERROR]) return;
@@ -603,10 +607,8 @@ __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
             const vec4 c0 = V.mvp * make4(p0[k], 1.0f);
             const vec4 c1 = V.mvp * make4(p1[k], 1.0f);
             const vec4 c2 = V.mvp * make4(p2[k], 1.0f);
-            // the host's bound put the mesh's bounding sphere inside every
-            // frustum plane (skip_clip): every face classifies as inside
             int cls = CLS_INSIDE;
-            if (!ka.skip_clip) cls = classify(c0, c1, c2);
+            if constexpr (!INSIDE) cls = classify(c0, c1, c2);
             want_clip = cls == CLS_CLIP;
             if (cls == CLS_INSIDE) {
                 r = make_rec(c0, c1, c2, V.W, V.H);
@@ -1983,7 +1985,10 @@ template <int T, bool BATCH>
 static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st, StageEvents* ev, const FbWait& wait) {
     t_phases = kVisPhases;
     const uint32_t vis_blocks = (ka.total_faces + 256u * VIS_FACES_PER_LANE - 1) / (256u * VIS_FACES_PER_LANE);
-    if (vis_blocks) stage_launch1(k_vis<T, BATCH>, dim3(vis_blocks), dim3(256), st, ev, 0, ka, ar);
+    if (vis_blocks) {
+        if (!BATCH && ka.skip_clip) stage_launch1(k_vis<T, false, true>, dim3(vis_blocks), dim3(256), st, ev, 0, ka, ar);
+        else stage_launch1(k_vis<T, BATCH>, dim3(vis_blocks), dim3(256), st, ev, 0, ka, ar);
+    }
     // passes the host showed to have no work are not launched
     if (BATCH || !ka.skip_clip) stage_launch1(k_clip<T, BATCH, true>, dim3(256), dim3(64), st, ev, 1, ka, ar);
     if (BATCH || !ka.skip_big) stage_launch1(k_big<T, BATCH>, dim3(1024), dim3(256), st, ev, 2, ka, ar);
