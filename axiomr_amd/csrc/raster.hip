@@ -1638,7 +1638,9 @@ __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
 // with an id plane (IDS) the slot keeps the winner's key in its low word
 // and the colour goes to the arena's colour slots (vcol).
 
-template <int T, int SK, bool IDS>
+// INSIDE: as for k_vis, every face lies inside the frustum (ka.skip_clip),
+// so the winners' classification is left out.
+template <int T, int SK, bool IDS, bool INSIDE = false>
 __global__ __launch_bounds__(256) void k_shade(KArgs ka, Arena ar) {
     using S = typename ShaderOf<SK>::type;
     if (ar.counters[CNT_ERROR]) return;
@@ -1669,7 +1671,7 @@ __global__ __launch_bounds__(256) void k_shade(KArgs ka, Arena ar) {
         if (k == EMPTY_KEY) continue;
         float z;
         uint32_t bgra;
-        const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra, ka.skip_clip != 0);
+        const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra, INSIDE);
         vk[pix] = drawn ? ((unsigned long long)__float_as_uint(z) << 32) | (IDS ? (uint32_t)k : bgra) : EMPTY_KEY;
         if (IDS && drawn) ar.vcol[V.pix_base + pix] = bgra;
     }
@@ -1953,8 +1955,10 @@ static auto by_shader(int shader) -> decltype(K<-1>::fn) {
     default: return K<-1>::fn;
     }
 }
-template <int T, bool IDS> struct ShadeK {
-    template <int SK> struct K { static constexpr auto fn = &k_shade<T, (SK < 0 ? SHADER_FLAT_COLOR : SK), IDS>; };
+template <int T, bool IDS, bool INSIDE> struct ShadeK {
+    template <int SK> struct K {
+        static constexpr auto fn = &k_shade<T, (SK < 0 ? SHADER_FLAT_COLOR : SK), IDS, INSIDE>;
+    };
 };
 template <int T, bool BATCH> struct ResolveK {
     template <int SK> struct K { static constexpr auto fn = &k_resolve<T, BATCH, SK>; };
@@ -1969,7 +1973,9 @@ static hipError_t launch_two_pass(const KArgs& ka, const Arena& ar, hipStream_t 
                                   const FbWait& wait) {
     const uint64_t units = (uint64_t)ka.total_tiles * (uint64_t)((T * T + 63) / 64);
     const int sb = (int)std::min<uint64_t>(std::max<uint64_t>(1, (units + 3) / 4), RESOLVE_MAX_BLOCKS);
-    stage_launch1(by_shader<ShadeK<T, IDS>::template K>(ka.shader), dim3(sb), dim3(256), st, ev, 3, ka, ar);
+    const auto shade = ka.skip_clip ? by_shader<ShadeK<T, IDS, true>::template K>(ka.shader)
+                                    : by_shader<ShadeK<T, IDS, false>::template K>(ka.shader);
+    stage_launch1(shade, dim3(sb), dim3(256), st, ev, 3, ka, ar);
     hipError_t e = fb_wait(st, wait);
     if (e != hipSuccess) return e;
     const uint32_t strips = (((uint32_t)ka.one.W + 63u) / 64u) * (uint32_t)ka.one.tiles_y;
