@@ -1423,7 +1423,9 @@ __global__ __launch_bounds__(256) void k_big(KArgs ka, Arena ar) {
 // Shades the winner of key k at (px, py) if it passes the depth test
 // against dfb (the framebuffer's depth, or the depth of a clear the draw
 // absorbs): returns whether it did, with its depth and packed colour.
-template <class S>
+// VARY: the view may carry the batch's vertex-stage outputs (V.vary); false
+// for plain draws, whose views never do.
+template <class S, bool VARY = true>
 __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
                                                 int py, float dfb, float& z, uint32_t& bgra, bool no_clip = false) {
     const uint32_t key = (uint32_t)k;
@@ -1437,7 +1439,7 @@ __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& 
     constexpr bool kVerts = sizeof(typename S::Var) > 1;
     Vertex56 v0, v1, v2;
     vec3 p0, p1, p2;
-    if (kVerts && !V.vary) {
+    if (kVerts && !(VARY && V.vary)) {
         v0 = gld(V.verts, i0);
         v1 = gld(V.verts, i1);
         v2 = gld(V.verts, i2);
@@ -1476,7 +1478,7 @@ __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& 
     if constexpr (kVerts) {
         if (clipped) {
             for (int q = 0; q < 3; ++q) var[q] = S::vs(ar.clip_verts[3 * (size_t)slot + q], V.u);
-        } else if (V.vary) {
+        } else if (VARY && V.vary) {
             const typename S::Var* vy = static_cast<const typename S::Var*>(V.vary);
             var[0] = gld(vy, i0);
             var[1] = gld(vy, i1);
@@ -1671,7 +1673,7 @@ __global__ __launch_bounds__(256) void k_shade(KArgs ka, Arena ar) {
         if (k == EMPTY_KEY) continue;
         float z;
         uint32_t bgra;
-        const bool drawn = shade_vis_pixel<S>(V, ar, k, px, py, ka.clear_depth, z, bgra, INSIDE);
+        const bool drawn = shade_vis_pixel<S, false>(V, ar, k, px, py, ka.clear_depth, z, bgra, INSIDE);
         vk[pix] = drawn ? ((unsigned long long)__float_as_uint(z) << 32) | (IDS ? (uint32_t)k : bgra) : EMPTY_KEY;
         if (IDS && drawn) ar.vcol[V.pix_base + pix] = bgra;
     }
