@@ -1495,12 +1495,13 @@ __device__ __forceinline__ bool shade_vis_pixel(const ViewDesc& V, const Arena& 
 
 // Returns whether the winner passed the depth test against dfb and was
 // written.
-template <class S>
+// VARY, INSIDE: as for shade_vis_pixel and k_shade.
+template <class S, bool VARY, bool INSIDE>
 __device__ __forceinline__ bool resolve_vis_pixel(const ViewDesc& V, const Arena& ar, unsigned long long k, int px,
                                                   int py, size_t pix, float dfb) {
     float z;
     uint32_t bgra;
-    if (!shade_vis_pixel<S>(V, ar, k, px, py, dfb, z, bgra)) return false;
+    if (!shade_vis_pixel<S, VARY>(V, ar, k, px, py, dfb, z, bgra, INSIDE)) return false;
     gst(V.depth, pix, z);
     gst(V.color, pix, bgra);
     if (V.ids) gst(V.ids, pix, (uint32_t)k);
@@ -1521,7 +1522,7 @@ __device__ __forceinline__ void clear_pixel(const ViewDesc& V, size_t pix, const
 }
 
 // Pixels p0, p0 + step, ... of the bin at (ox, oy), row-major.
-template <int T, class S>
+template <int T, class S, bool VARY, bool INSIDE>
 __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena& ar, int ox, int oy,
                                                  const ClearSpec& cs, int p0, int step) {
     unsigned long long* vk = ar.vkeys + V.pix_base;
@@ -1533,7 +1534,7 @@ __device__ __forceinline__ void resolve_vis_tile(const ViewDesc& V, const Arena&
         bool drawn = false;
         if (k != EMPTY_KEY) {
             vk[pix] = EMPTY_KEY;
-            drawn = resolve_vis_pixel<S>(V, ar, k, px, py, pix, cs.on ? cs.depth : gld(V.depth, pix));
+            drawn = resolve_vis_pixel<S, VARY, INSIDE>(V, ar, k, px, py, pix, cs.on ? cs.depth : gld(V.depth, pix));
         }
         if (!drawn && cs.on) clear_pixel(V, pix, cs);
     }
@@ -1551,19 +1552,19 @@ __device__ __forceinline__ void clear_vis_tile(const ViewDesc& V, int ox, int oy
     }
 }
 
-template <int T, int SK>
+template <int T, int SK, bool VARY, bool INSIDE>
 __device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const Arena& ar, int ox, int oy,
                                                       const ClearSpec& cs, int p0, int step) {
     if constexpr (SK >= 0) {
-        resolve_vis_tile<T, typename ShaderOf<SK>::type>(V, ar, ox, oy, cs, p0, step);
+        resolve_vis_tile<T, typename ShaderOf<SK>::type, VARY, INSIDE>(V, ar, ox, oy, cs, p0, step);
     } else {
         switch (V.u.kind) {
-        case SHADER_FLAT: resolve_vis_tile<T, FlatShader>(V, ar, ox, oy, cs, p0, step); break;
-        case SHADER_PHONG: resolve_vis_tile<T, PhongShader>(V, ar, ox, oy, cs, p0, step); break;
-        case SHADER_PBR: resolve_vis_tile<T, PBRShader>(V, ar, ox, oy, cs, p0, step); break;
-        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader>(V, ar, ox, oy, cs, p0, step); break;
-        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader>(V, ar, ox, oy, cs, p0, step); break;
-        default: resolve_vis_tile<T, FlatColorShader>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_FLAT: resolve_vis_tile<T, FlatShader, VARY, INSIDE>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_PHONG: resolve_vis_tile<T, PhongShader, VARY, INSIDE>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_PBR: resolve_vis_tile<T, PBRShader, VARY, INSIDE>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_GOURAUD: resolve_vis_tile<T, GouraudShader, VARY, INSIDE>(V, ar, ox, oy, cs, p0, step); break;
+        case SHADER_PHONG_POINT: resolve_vis_tile<T, PhongPointShader, VARY, INSIDE>(V, ar, ox, oy, cs, p0, step); break;
+        default: resolve_vis_tile<T, FlatColorShader, VARY, INSIDE>(V, ar, ox, oy, cs, p0, step); break;
         }
     }
 }
@@ -1581,7 +1582,10 @@ __device__ __forceinline__ void resolve_vis_tile_kind(const ViewDesc& V, const A
 // against the clear depth without reading the framebuffer.  If the frame
 // failed (an arena overflow), only the clear is written, as a separate
 // clear pass would have.
-template <int T, bool BATCH, int SK>
+// INSIDE (plain draws only): every face lies inside the frustum
+// (ka.skip_clip), so the winners' classification is left out.  Only a
+// batch's views carry vertex-stage outputs (VARY = BATCH).
+template <int T, bool BATCH, int SK, bool INSIDE = false>
 __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
     const bool failed = ar.counters[CNT_ERROR] != 0u;
     const ClearSpec cs{ka.clear != 0, ka.clear_bgra, ka.clear_depth};
@@ -1606,7 +1610,7 @@ __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
             const ViewDesc& V = view_at<BATCH>(ka, vi);
             const uint32_t local = t - V.tile_base;
             const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
-            if (keys) resolve_vis_tile_kind<T, SK>(V, ar, ox, oy, cs, (int)__lane_id(), 64);
+            if (keys) resolve_vis_tile_kind<T, SK, BATCH, INSIDE>(V, ar, ox, oy, cs, (int)__lane_id(), 64);
             else clear_vis_tile<T>(V, ox, oy, cs, (int)__lane_id(), 64);
         }
     } else {
@@ -1623,7 +1627,7 @@ __global__ __launch_bounds__(256) void k_resolve(KArgs ka, Arena ar) {
             const ViewDesc& V = view_at<BATCH>(ka, vi);
             const uint32_t local = t - V.tile_base;
             const int ox = (int)(local % (uint32_t)V.tiles_x) * T, oy = (int)(local / (uint32_t)V.tiles_x) * T;
-            if (keys) resolve_vis_tile_kind<T, SK>(V, ar, ox, oy, cs, (int)threadIdx.x, 256);
+            if (keys) resolve_vis_tile_kind<T, SK, BATCH, INSIDE>(V, ar, ox, oy, cs, (int)threadIdx.x, 256);
             else clear_vis_tile<T>(V, ox, oy, cs, (int)threadIdx.x, 256);
         }
     }
@@ -1962,8 +1966,8 @@ template <int T, bool IDS, bool INSIDE> struct ShadeK {
         static constexpr auto fn = &k_shade<T, (SK < 0 ? SHADER_FLAT_COLOR : SK), IDS, INSIDE>;
     };
 };
-template <int T, bool BATCH> struct ResolveK {
-    template <int SK> struct K { static constexpr auto fn = &k_resolve<T, BATCH, SK>; };
+template <int T, bool BATCH, bool INSIDE> struct ResolveK {
+    template <int SK> struct K { static constexpr auto fn = &k_resolve<T, BATCH, SK, INSIDE>; };
 };
 template <int T, int NT, bool BATCH> struct RasterK {
     template <int SK> struct K { static constexpr auto fn = &k_raster<T, NT, BATCH, SK>; };
@@ -2011,7 +2015,9 @@ static hipError_t launch_vis_t(const KArgs& ka, const Arena& ar, hipStream_t st,
     // T = 8: four bins per workgroup; larger bins: one bin per workgroup
     const uint32_t per_block = T == 8 ? 4u : 1u;
     const int rb = imax(1, imin((int)((ka.total_tiles + per_block - 1) / per_block), RESOLVE_MAX_BLOCKS));
-    stage_launch1(by_shader<ResolveK<T, BATCH>::template K>(ka.shader), dim3(rb), dim3(256), st, ev, 4, ka, ar);
+    const auto resolve = (!BATCH && ka.skip_clip) ? by_shader<ResolveK<T, false, true>::template K>(ka.shader)
+                                                  : by_shader<ResolveK<T, BATCH, false>::template K>(ka.shader);
+    stage_launch1(resolve, dim3(rb), dim3(256), st, ev, 4, ka, ar);
     return hipGetLastError();
 }
 
