@@ -245,6 +245,16 @@ struct KArgs {
     int32_t skip_big = 0;       // frustum plane, so no face is classified and the
                                 // clipper is not launched (nor, when no triangle can
                                 // exceed the footprint, the large-triangle pass)
+    // skip_clip views: k_vis's back-face pre-test in object space.  Every w
+    // is then positive, so a face's screen-space winding has the sign of
+    // cull_sign * ((p0 - cull_eye) . ((p1 - p0) x (p2 - p0))); a face whose
+    // value lies below -max(cull_tau, 1e-3 |p0 - eye|_1 |p1 - p0|_1 |p2 - p0|_1)
+    // covers more than 0.25 px^2 at the largest w of the mesh and is culled
+    // by the exact screen-space test as well, so it is skipped before the
+    // transform (cull_sign 0: no pre-test).
+    float cull_eye[3] = {0.0f, 0.0f, 0.0f};
+    float cull_sign = 0.0f;
+    float cull_tau = 0.0f;
     int32_t depth_is_clear = 0; // binned mode: every framebuffer's depth plane holds
                                 // clear_depth (written just before, by k_clear /
                                 // k_clear_views): winners are tested against that

@@ -462,6 +462,37 @@ PassSkips pass_skips(const axr_mesh* m, const mat4& mvp, int W, int H, int footp
         if (!(at_center(q) - slope(q) * m->radius > margin(q))) return s;
     }
     s.clip = true;
+    // The back-face pre-test (k_vis): the eye in object space is where rows
+    // x, y and w of the transform vanish, A3 e + a = 0 (A3: those rows'
+    // first three columns, a: their last).  Then (x, y, w) of a vertex is
+    // A3 (p - e), and a face's screen-space cross product is
+    // (W H / 4) det(A3) ((p0 - e) . n) / (w0 w1 w2), n = (p1 - p0) x (p2 - p0).
+    {
+        const double* R[3] = {r[0], r[1], r[3]};
+        auto det3 = [](const double (&m)[3][3]) {
+            return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                   m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+        };
+        double A[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) A[i][j] = R[i][j];
+        const double det = det3(A);
+        const double w_hi = at_center(r[3]) + slope(r[3]) * m->radius;
+        if (std::isfinite(det) && det != 0.0 && w_hi > 0.0) {
+            for (int j = 0; j < 3; ++j) {   // Cramer's rule for A3 e = -a
+                double Aj[3][3];
+                for (int i = 0; i < 3; ++i)
+                    for (int k = 0; k < 3; ++k) Aj[i][k] = k == j ? -R[i][3] : A[i][k];
+                s.eye[j] = det3(Aj) / det;
+            }
+            s.sign = det > 0.0 ? 1.0 : -1.0;
+            // |(p0 - e) . n| above tau: the screen cross product exceeds
+            // 0.25 px^2 (w <= w_hi over the sphere), some 40 times the float
+            // error of the kernels' screen-space test for faces of a few pixels
+            s.tau = w_hi * w_hi * w_hi / ((double)W * (double)H * std::fabs(det));
+            if (!std::isfinite(s.tau) || !std::isfinite(s.eye[0] + s.eye[1] + s.eye[2])) s.sign = 0.0;
+        }
+    }
     // Screen extent of any face: for p, q within max_edge of each other in
     // the sphere, |x_ndc(p) - x_ndc(q)| <= max_edge * sup |grad(a.P / b.P)|
     // over the segment, with grad = (a' (b.P) - b' (a.P)) / (b.P)^2.  The
@@ -1056,6 +1087,11 @@ int axr_draw(axr_ctx* ctx, axr_fb* fb, const axr_mesh* mesh, const axr_draw_para
         const PassSkips sk = pass_skips(mesh, ka.one.mvp, fb->w, fb->h, ka.small_footprint);
         ka.skip_clip = sk.clip;
         ka.skip_big = sk.big && ka.mode == RASTER_VISIBILITY;
+        if (sk.clip && sk.sign != 0.0) {
+            for (int j = 0; j < 3; ++j) ka.cull_eye[j] = (float)sk.eye[j];
+            ka.cull_sign = (float)sk.sign;
+            ka.cull_tau = (float)sk.tau;
+        }
         ctx->stats.passes_skipped += (uint64_t)ka.skip_clip + (uint64_t)ka.skip_big;
     }
     if (fb->clear_pending && ka.mode == RASTER_VISIBILITY) {

@@ -214,6 +214,9 @@ int uniform_shader(const ViewDesc* views, uint32_t n);
 struct PassSkips {
     bool clip = false;   // no face crosses a frustum plane
     bool big = false;    // and no triangle's pixel rectangle exceeds `footprint`
+    // with clip: the back-face pre-test's constants (KArgs::cull_*)
+    double eye[3] = {0.0, 0.0, 0.0};
+    double sign = 0.0, tau = 0.0;
 };
 PassSkips pass_skips(const axr_mesh* m, const mat4& mvp, int W, int H, int footprint);
 ViewDesc make_view(const axr_mesh* m, uint32_t* color, float* depth, uint32_t* ids, int w, int h,

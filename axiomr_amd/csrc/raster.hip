@@ -597,6 +597,24 @@ __global__ __launch_bounds__(256) void k_vis(KArgs ka, Arena ar) {
             p2[k] = load_pos(V.pos, i2[k]);
         }
     }
+    if constexpr (INSIDE) {
+        // faces clearly facing away from the eye are culled before the
+        // transform (the pre-test of KArgs::cull_sign; a face within its
+        // margin goes on to the exact screen-space test)
+        if (ka.cull_sign != 0.0f) {
+            const vec3 e = make3(ka.cull_eye[0], ka.cull_eye[1], ka.cull_eye[2]);
+            const float le = fabsf(e.x) + fabsf(e.y) + fabsf(e.z);
+#pragma unroll
+            for (uint32_t k = 0; k < F; ++k) {
+                if (!act[k]) continue;
+                const vec3 a = p0[k] - e, b = p1[k] - p0[k], c = p2[k] - p0[k];
+                const float d = ka.cull_sign * dot(a, cross(b, c));
+                const float la = fabsf(a.x) + fabsf(a.y) + fabsf(a.z) + le;
+                const float lb = fabsf(b.x) + fabsf(b.y) + fabsf(b.z), lc = fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
+                if (d < -fmaxf(ka.cull_tau, 1e-3f * la * lb * lc)) act[k] = false;
+            }
+        }
+    }
 #pragma unroll
     for (uint32_t k = 0; k < F; ++k) {
         bool want_clip = false, big = false;
