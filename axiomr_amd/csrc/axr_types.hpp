@@ -98,7 +98,8 @@ struct ViewDesc {
     uint32_t face_base;            // first slot of this view in the record array
     uint32_t tile_base;            // first bin of this view
     uint32_t pix_base;             // first pixel of this view in the visibility buffer
-    uint32_t pad;
+    uint32_t inside;               // batches: the mesh's bounding sphere lies inside every
+                                   // frustum plane, so k_setup leaves the classification out
 };
 
 // Triangle record, 48 bytes (three 16-byte loads).
@@ -241,9 +242,10 @@ struct KArgs {
     const uint32_t* vary_firsts = nullptr;
     uint32_t vary_njobs = 0;
     uint32_t vary_total = 0;
-    int32_t skip_clip = 0;      // one view: the mesh's bounding sphere lies inside every
-    int32_t skip_big = 0;       // frustum plane, so no face is classified and the
-                                // clipper is not launched (nor, when no triangle can
+    int32_t skip_clip = 0;      // one view, or every view of a binned batch: the mesh's
+    int32_t skip_big = 0;       // bounding sphere lies inside every frustum plane, so
+                                // no face is classified and the clipper is not
+                                // launched (one view: nor, when no triangle can
                                 // exceed the footprint, the large-triangle pass)
     // skip_clip views: k_vis's back-face pre-test in object space.  Every w
     // is then positive, so a face's screen-space winding has the sign of

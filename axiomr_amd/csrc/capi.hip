@@ -1191,6 +1191,7 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     std::vector<uint32_t> fbase(nviews + 1), tbase(nviews + 1), pbase(nviews + 1);
     uint64_t faces = 0, tiles = 0, pixels = 0;
     const int tile = ctx->cfg.tile_size;
+    bool all_inside = true;
     // every view writes its framebuffer in the same launch, with no order
     // between views: two views of one framebuffer would race on its pixels
     // (the reference's render loop draws meshes into one framebuffer one
@@ -1213,6 +1214,8 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
         }
         const axr_fb* fb = fbs[i];
         ViewDesc v = make_view(meshes[i], fb->color, fb->depth, fb->ids, fb->w, fb->h, &params[i], tile);
+        v.inside = !ctx->no_pass_skips && pass_skips(meshes[i], v.mvp, fb->w, fb->h, ctx->cfg.small_footprint).clip;
+        all_inside = all_inside && v.inside;
         v.face_base = (uint32_t)faces;
         v.tile_base = (uint32_t)tiles;
         v.pix_base = (uint32_t)pixels;
@@ -1288,6 +1291,9 @@ int axr_batch_create(axr_ctx* ctx, uint32_t nviews, axr_fb* const* fbs, const ax
     ka.total_pixels = (uint32_t)pixels;
     ka.shader = uniform_shader(b->views.data(), nviews);
     ka.one = b->views[0];
+    // binned mode: every view inside its frustum, so neither the setup's
+    // classification nor the clipper is run
+    ka.skip_clip = all_inside && ka.mode != RASTER_VISIBILITY;
     b->fs = frame_size(ka, (uint32_t)ctx->cfg.bin_capacity);
     if ((r = arena_ensure(ctx, err, ctx->main, b->fs))) { axr_batch_destroy(b); return r; }
     // the batch's tables and captured graph hold the views' device pointers:

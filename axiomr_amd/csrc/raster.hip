@@ -489,7 +489,7 @@ __device__ __forceinline__ void big_append_lane(const Arena& ar, const TriRec& r
 // k_setup: one lane per face (submission order across views).
 
 // Face slot g of view V (all 64 lanes of the wave call it).
-template <int T>
+template <int T, bool INSIDE>
 __device__ __forceinline__ void setup_face(const KArgs& ka, const Arena& ar, uint32_t g, const ViewDesc& V) {
     bool active = g < ka.total_faces;
     bool want_clip = false, binnable = false;
@@ -498,7 +498,8 @@ __device__ __forceinline__ void setup_face(const KArgs& ka, const Arena& ar, uin
         const uint32_t s = g - V.face_base;
         FaceGeo f;
         face_geo(V, s, f);
-        const int cls = classify(f.c0, f.c1, f.c2);
+        int cls = CLS_INSIDE;
+        if constexpr (!INSIDE) cls = classify(f.c0, f.c1, f.c2);
         want_clip = cls == CLS_CLIP;
         if (cls == CLS_INSIDE) {
             TriRec r = make_rec(f.c0, f.c1, f.c2, V.W, V.H);
@@ -526,7 +527,11 @@ __device__ __forceinline__ void setup_face(const KArgs& ka, const Arena& ar, uin
     emit_pairs_wave<T>(ar, V, binnable, rc, g, g >> 6);
 }
 
-template <int T, bool BATCH>
+// INSIDE: the host's bound put every view's mesh inside every frustum plane
+// (ka.skip_clip), so the classification is left out, as in k_vis.  A batch
+// wave whose faces lie in one view leaves it out for a view the bound puts
+// inside (ViewDesc::inside), a branch on a value the wave holds in common.
+template <int T, bool BATCH, bool INSIDE = false>
 __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
     if (ar.counters[CNT_ERROR]) return;
@@ -537,12 +542,13 @@ __global__ __launch_bounds__(256) void k_setup(KArgs ka, Arena ar) {
             // the whole wave in one view (the usual case): the view's fields
             // in scalar registers, loaded once for the wave
             const ViewDesc V = wave_uniform(ka.table[v0]);
-            setup_face<T>(ka, ar, g, V);
+            if (INSIDE || V.inside) setup_face<T, true>(ka, ar, g, V);
+            else setup_face<T, false>(ka, ar, g, V);
         } else {
-            setup_face<T>(ka, ar, g, ka.table[vi]);
+            setup_face<T, INSIDE>(ka, ar, g, ka.table[vi]);
         }
     } else {
-        setup_face<T>(ka, ar, g, ka.one);
+        setup_face<T, INSIDE>(ka, ar, g, ka.one);
     }
 }
 
@@ -2050,8 +2056,11 @@ static hipError_t launch_frame_t(const KArgs& ka, const Arena& ar, hipStream_t s
     t_phases = kBinnedPhases;
     const uint32_t ntiles = ka.total_tiles;
     const uint32_t setup_blocks = (ka.total_faces + 255) / 256;
-    if (setup_blocks) stage_launch1(k_setup<T, BATCH>, dim3(setup_blocks), dim3(256), st, ev, 0, ka, ar);
-    if (BATCH || !ka.skip_clip) stage_launch1(k_clip<T, BATCH, false>, dim3(256), dim3(64), st, ev, 1, ka, ar);
+    if (setup_blocks) {
+        if (ka.skip_clip) stage_launch1(k_setup<T, BATCH, true>, dim3(setup_blocks), dim3(256), st, ev, 0, ka, ar);
+        else stage_launch1(k_setup<T, BATCH>, dim3(setup_blocks), dim3(256), st, ev, 0, ka, ar);
+    }
+    if (!ka.skip_clip) stage_launch1(k_clip<T, BATCH, false>, dim3(256), dim3(64), st, ev, 1, ka, ar);
     if (ntiles <= SMALL_SCAN_TILES) {
         stage_launch1(k_scan_single, dim3(1), dim3(SCAN_THREADS), st, ev, 2, ar, ntiles);
     } else {
