@@ -44,3 +44,42 @@ def test_skips_keep_every_pixel(ctx, case):
         assert skipped == (2 if mode == N.RASTER_VISIBILITY else 1), (case, skipped)
     if case in ("near_plane", "inside", "soup"):
         assert skipped == 0, (case, skipped)
+
+
+@pytest.mark.parametrize("crossing", [False, True])
+def test_batch_of_inside_views_equals_solo(ctx, crossing):
+    """A binned batch whose every view the bounds put inside the frustum runs
+    its setup without the classification and launches no clipper; one view
+    that crosses the near plane brings the clipper back for the batch and the
+    classification for its own faces.  Either way every view equals its solo
+    draw, and the crossing view equals the checker."""
+    import dataclasses
+    wl = S.batch_workload(nviews=10, nmeshes=4, fmin=500, fmax=20_000, wmin=64, wmax=320, hmin=48, hmax=240,
+                          tex=64, seed=11)
+    for i, v in enumerate(wl.views):
+        p = np.asarray(v.camera_pos, np.float64)
+        d = 1.05 if (crossing and i == 3) else 4.0   # 1.05: on the displaced surface, faces cross the near plane
+        wl.views[i] = dataclasses.replace(v, camera_pos=tuple(float(x) for x in p * (d / np.linalg.norm(p))))
+    gb = H.GpuBatch(ctx, wl)
+    try:
+        ctx.set_profiling(True)         # resets the counters; kernels timed per stage
+        gb.batch.launch()
+        ctx.sync()
+        st = ctx.stats()
+        ctx.set_profiling(False)
+        assert st.skip_faults == 0
+        clip_launched = st.stage_ms_total[1] > 0     # stage 1: k_clip in both modes
+        if ctx.config()["raster_mode"] != N.RASTER_VISIBILITY:
+            assert clip_launched == crossing, (crossing, list(st.stage_ms_total))
+        gb.batch.launch()               # the captured graph
+        ctx.sync()
+        batched = [fb.read() for fb in gb.fbs]
+        for i in range(len(wl.views)):
+            gb.draw_solo(i)
+            solo = gb.fbs[i].read()
+            for a, b in zip(batched[i], solo):
+                assert (a.view(np.uint8) == b.view(np.uint8)).all(), f"view {i} differs from its solo render"
+        if crossing:
+            H.assert_parity(H.compare(batched[3], H.render_oracle(wl.scene(3))), "crossing view")
+    finally:
+        gb.close()
