@@ -45,11 +45,19 @@ template <class T>
 __device__ __forceinline__ void gst(T* p, size_t i, const T& v) {
     __builtin_memcpy((__attribute__((address_space(1))) T*)p + i, &v, sizeof(T));
 }
+// gst with the non-temporal hint (a cache hint only) for a scalar or native
+// vector T: framebuffer writes that the frame's later passes do not read.
+template <class T>
+__device__ __forceinline__ void gst_nt(T* p, size_t i, T v) {
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)p + i);
+}
 #else
 template <class T>
 AXR_HD T gld(const T* p, size_t i) { return p[i]; }
 template <class T>
 AXR_HD void gst(T* p, size_t i, const T& v) { p[i] = v; }
+template <class T>
+AXR_HD void gst_nt(T* p, size_t i, T v) { p[i] = v; }
 #endif
 
 struct vec2 { float x, y; };

@@ -1084,9 +1084,9 @@ __device__ __forceinline__ void resolve_pixel(const ViewDesc& V, const Arena& ar
     const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
     if (!(z < (known ? known_depth : gld(V.depth, pix)))) return;   // reference mergeTileResults: strict less
     const vec4 col = shade_winner<S>(V, ar, r, slot, clipped, rank, make3(al, be, ga), um);
-    gst(V.depth, pix, z);
-    gst(V.color, pix, pack_bgra(col));
-    if (V.ids) gst(V.ids, pix, key);
+    gst_nt(V.depth, pix, z);
+    gst_nt(V.color, pix, pack_bgra(col));
+    if (V.ids) gst_nt(V.ids, pix, key);
 }
 
 template <int T, int NT, class S>
@@ -1827,9 +1827,11 @@ __global__ __launch_bounds__(256) void k_finish4(KArgs ka, Arena ar) {
                 q[0] = ev2;
                 q[1] = ev2;
             }
-            *reinterpret_cast<uint4*>(V.color + pix) = c;
-            *reinterpret_cast<float4*>(V.depth + pix) = d;
-            if constexpr (IDS) *reinterpret_cast<uint4*>(V.ids + pix) = id;
+            typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+            gst_nt(reinterpret_cast<v4u*>(V.color + pix), 0, v4u{c.x, c.y, c.z, c.w});
+            gst_nt(reinterpret_cast<v4u*>(V.depth + pix), 0,
+                   v4u{__float_as_uint(d.x), __float_as_uint(d.y), __float_as_uint(d.z), __float_as_uint(d.w)});
+            if constexpr (IDS) gst_nt(reinterpret_cast<v4u*>(V.ids + pix), 0, v4u{id.x, id.y, id.z, id.w});
         }
         // every lane of a bin read its flag before the loop's loads returned
         if (keys && rg == 0u && (uint32_t)px % (uint32_t)T == 0u) ar.tile_count[t] = 0u;
@@ -1844,16 +1846,18 @@ __global__ __launch_bounds__(256) void k_finish4(KArgs ka, Arena ar) {
 __device__ __forceinline__ void clear_planes(uint32_t* color, float* depth, uint32_t* ids, size_t n, uint32_t bgra,
                                              float d, size_t first, size_t stride) {
     const size_t n4 = n / 4;
-    uint4* c4 = reinterpret_cast<uint4*>(color);
-    float4* d4 = reinterpret_cast<float4*>(depth);
-    uint4* i4 = reinterpret_cast<uint4*>(ids);
-    const uint4 cv = make_uint4(bgra, bgra, bgra, bgra);
-    const float4 dv = make_float4(d, d, d, d);
-    const uint4 iv = make_uint4(~0u, ~0u, ~0u, ~0u);
+    // non-temporal 16-byte stores (a cache hint only): a clear streams whole
+    // planes, which later passes read back at most once
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    v4u* c4 = reinterpret_cast<v4u*>(color);
+    v4u* d4 = reinterpret_cast<v4u*>(depth);
+    v4u* i4 = reinterpret_cast<v4u*>(ids);
+    const uint32_t du = __float_as_uint(d);
+    const v4u cv = {bgra, bgra, bgra, bgra}, dv = {du, du, du, du}, iv = {~0u, ~0u, ~0u, ~0u};
     for (size_t i = first; i < n4; i += stride) {
-        c4[i] = cv;
-        d4[i] = dv;
-        if (ids) i4[i] = iv;
+        __builtin_nontemporal_store(cv, c4 + i);
+        __builtin_nontemporal_store(dv, d4 + i);
+        if (ids) __builtin_nontemporal_store(iv, i4 + i);
     }
     for (size_t i = 4 * n4 + first; i < n; i += stride) {
         color[i] = bgra;
