@@ -489,6 +489,16 @@ __device__ __forceinline__ void big_append_lane(const Arena& ar, const TriRec& r
 // k_setup: one lane per face (submission order across views).
 
 // Face slot g of view V (all 64 lanes of the wave call it).
+// A triangle record stored with the non-temporal hint: the batch's records
+// far exceed the caches and are read back only after the scatter.
+__device__ __forceinline__ void store_rec_nt(TriRec* p, const TriRec& r) {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    v4u* q = reinterpret_cast<v4u*>(p);
+    gst_nt(q, 0, v4u{__float_as_uint(r.a.x), __float_as_uint(r.a.y), __float_as_uint(r.a.z), __float_as_uint(r.a.w)});
+    gst_nt(q, 1, v4u{__float_as_uint(r.b.x), __float_as_uint(r.b.y), __float_as_uint(r.b.z), __float_as_uint(r.b.w)});
+    gst_nt(q, 2, v4u{r.c.x, r.c.y, r.c.z, r.c.w});
+}
+
 template <int T, bool INSIDE>
 __device__ __forceinline__ void setup_face(const KArgs& ka, const Arena& ar, uint32_t g, const ViewDesc& V) {
     bool active = g < ka.total_faces;
@@ -511,7 +521,7 @@ __device__ __forceinline__ void setup_face(const KArgs& ka, const Arena& ar, uin
                         r.c.y = s * 8u;
                         r.c.z = f.face;
                         r.c.w = f.G.material & REC_MATERIAL_MASK;
-                        ar.rec[g] = r;
+                        store_rec_nt(ar.rec + g, r);
                         binnable = true;
                     }
                 }
